@@ -1,0 +1,129 @@
+// dev.h -- host-callable launchers for the HIP kernels in kernels.hip.
+//
+// Everything here runs asynchronously on the given stream.  Pointers are
+// device pointers unless named *_h.  Vector blocks are stored row-major
+// ("interleaved"): entry (row r, column c) of a B-column block lives at
+// x[r * B + c], so the B values a sparse row gathers sit in one contiguous
+// segment.  Lanczos bases are column-major with leading dimension ld.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace evsl_dev {
+
+// Row blocks for the LDS-staged CSR kernels: block b covers rows
+// [rb[b], rb[b+1]) and either holds at most kNnzCap nonzeros and kRowsCap
+// rows, or is a single row longer than kNnzCap.
+constexpr int kWG = 256;
+constexpr int kNnzCap = 2048;
+constexpr int kRowsCap = 256;
+
+struct Csr {
+  int n = 0, nnz = 0;
+  const int* ia = nullptr;
+  const int* ja = nullptr;
+  const double* a64 = nullptr;
+  const float* a32 = nullptr;
+  const int* rb = nullptr;  // row-block starts, nblk + 1 entries
+  int nblk = 0;
+};
+
+enum Mode { kPlain = 0, kCheb = 1, kKpm = 2 };
+
+// One Chebyshev degree step over a B-column block (B in {1,2,4,8,16,32}):
+//   acc = A * xk
+//   kPlain: xkp1 = acc
+//   kCheb : w = t*(acc - cc*xk) - (xkm1 ? xkm1 : 0); xkp1 = w;
+//           y = (first ? mu0[c]*xk : y) + s[c]*w
+//   kKpm  : w as kCheb; xkp1 = w; part[blk*B + c] = sum_rows w*v0
+// s, mu0: per-column coefficients (device, B entries each).
+// fp32 selects float vectors and matrix values (float accumulation).
+hipError_t cheb_spmm(const Csr& A, int B, bool fp32, Mode mode, const void* xk,
+                     const void* xkm1, void* xkp1, void* y, const void* v0,
+                     const double* s, const double* mu0, double t, double cc,
+                     bool first, double* part, hipStream_t st);
+
+// out[c] = sum_b part[b*B + c] over nblk row blocks (one small launch)
+hipError_t sum_block_partials(const double* part, int nblk, int B, double* out,
+                              hipStream_t st);
+
+// Fixed-grid reductions: partial buffers hold kRedBlocks entries.
+constexpr int kRedBlocks = 1024;
+// part[b] = partial sum of x.y
+hipError_t dot_partial(size_t n, const double* x, const double* y, double* part,
+                       hipStream_t st);
+// out = sum part[0..cnt)  (single block)
+hipError_t sum_scalar(const double* part, int cnt, double* out, hipStream_t st);
+// y += a*x (a on host); part = partial of y.z   (z may be null: no dot)
+hipError_t axpy_dot(size_t n, double a, const double* x, double* y, const double* z,
+                    double* part, hipStream_t st);
+// y += sign * (*a_d) * x; part = partial of y.y
+hipError_t axpy_dev_nrm(size_t n, double sign, const double* a_d, const double* x,
+                        double* y, double* part, hipStream_t st);
+// y += sign * (*a_d) * x; part = partial of y.z
+hipError_t axpy_dev_dot(size_t n, double sign, const double* a_d, const double* x,
+                        double* y, const double* z, double* part, hipStream_t st);
+// x *= a
+hipError_t scal(size_t n, double a, double* x, hipStream_t st);
+// y = a * x
+hipError_t scal_copy(size_t n, double a, const double* x, double* y, hipStream_t st);
+// x[i] = v
+hipError_t fill(size_t n, double v, double* x, hipStream_t st);
+hipError_t fill_u32(size_t n, unsigned v, unsigned* x, hipStream_t st);
+// Z (k x k, column-major) = identity
+hipError_t set_identity(int k, double* Z, hipStream_t st);
+// precision conversions
+hipError_t f64_to_f32(size_t n, const double* x, float* y, hipStream_t st);
+hipError_t f32_to_f64(size_t n, const float* x, double* y, hipStream_t st);
+
+// Classical Gram-Schmidt pass against the k columns of V (column-major, ld):
+//   h = V^T w          (gemv_t: partials part[k][nch] then reduce)
+//   w -= V h, part2 = partial of ||w||^2
+int gemv_t_chunks(size_t n);
+hipError_t gemv_t(size_t n, int k, const double* V, size_t ld, const double* w,
+                  double* part, double* h, hipStream_t st);
+hipError_t gemv_n_sub_nrm(size_t n, int k, const double* V, size_t ld, const double* h,
+                          double* w, double* part, hipStream_t st);
+// y = V h (no subtraction)
+hipError_t gemv_n(size_t n, int k, const double* V, size_t ld, const double* h, double* y,
+                  hipStream_t st);
+
+// C(n x m) = A(n x k) * B(k x m), all column-major, FP64
+hipError_t gemm_nn(size_t n, int m, int k, const double* A, size_t lda, const double* B,
+                   size_t ldb, double* C, size_t ldc, hipStream_t st);
+
+// per-column reductions over m columns of n rows (column-major):
+//   col_dot:  out[j] = X(:,j).Y(:,j)
+//   col_res:  out[j] = || W(:,j) - t[j] U(:,j) ||^2
+hipError_t col_dot(size_t n, int m, const double* X, size_t ldx, const double* Y,
+                   size_t ldy, double* part, double* out, hipStream_t st);
+hipError_t col_res2(size_t n, int m, const double* W, size_t ldw, const double* U,
+                    size_t ldu, const double* t, double* part, double* out,
+                    hipStream_t st);
+// X(:,j) *= f(s[j]) with f = 1/sqrt(s) (normalise by squared norms)
+hipError_t col_scale_rsqrt(size_t n, int m, double* X, size_t ldx, const double* s2,
+                           hipStream_t st);
+// dst(:,j) = src(:, idx[j]) for j < m  (rows nrow)
+hipError_t gather_cols(size_t nrow, int m, const double* src, size_t lds, const int* idx,
+                       double* dst, size_t ldd, hipStream_t st);
+
+// Apply the plane rotations recorded by the implicit-QL tridiagonal solver to
+// the k x k column-major matrix Z (thread per row).  Sweep s covers rotation
+// indices i = top[s], top[s]-1, ..., top[s]-cnt[s]+1 with cosines/sines
+// cs[2*(off[s]+j)], cs[2*(off[s]+j)+1].
+hipError_t apply_rotations(int k, double* Z, int nsweep, const int* top, const int* cnt,
+                           const long long* off, const double* cs, hipStream_t st);
+
+// block (row-major, B columns) helpers
+// out[c] = ||X(:,c)||^2 partials -> finalised into out (B values)
+hipError_t rm_col_nrm2(size_t n, int B, const double* X, double* part, double* out,
+                       hipStream_t st);
+// X(:,c) *= 1/sqrt(s2[c]) (zero columns left zero)
+hipError_t rm_col_scale_rsqrt(size_t n, int B, double* X, const double* s2,
+                              hipStream_t st);
+// x(r*B+c) for r<n, c<B from Rademacher / Gaussian streams keyed by (seed, col0+c, r)
+hipError_t rm_random(size_t n, int B, int kind, unsigned long long seed, int col0,
+                     int ncols, double* X, hipStream_t st);
+
+}  // namespace evsl_dev

@@ -1,0 +1,140 @@
+// internal.h -- host-side state shared by the C-ABI entry points.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/evsl.h"
+#include "../../include/evsl_amd.h"
+#include "dev.h"
+
+namespace evsl {
+
+// ---- error handling ------------------------------------------------------
+// HIP failures are recorded (first message kept) and turned into
+// EVSL_ERR_DEVICE by the entry points; nothing calls exit() or assert().
+void set_error(const char* where, hipError_t e);
+void set_error_msg(const std::string& msg);
+const char* last_error();
+
+#define EVSL_HIP(call)                                   \
+  do {                                                   \
+    hipError_t e_ = (call);                              \
+    if (e_ != hipSuccess) {                              \
+      ::evsl::set_error(#call, e_);                      \
+      return EVSL_ERR_DEVICE;                            \
+    }                                                    \
+  } while (0)
+
+// ---- device buffers ----------------------------------------------------
+// RAII device allocation used inside the solvers.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  hipError_t alloc(size_t nbytes);
+  void release();
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// ---- operator ----------------------------------------------------------
+struct DevMatrix {
+  int n = 0, nnz = 0;
+  int *ia = nullptr, *ja = nullptr, *rb = nullptr;
+  double* a64 = nullptr;
+  float* a32 = nullptr;
+  int nblk = 0;
+  bool owns_csr = false;  // ia/ja/a64 allocated here (SetAMatrix)
+  evsl_dev::Csr view() const;
+  void release();
+};
+
+// Build the LDS row-block partition from host row pointers.
+std::vector<int> make_row_blocks(const int* ia_h, int n);
+
+struct Context {
+  bool started = false;
+  int n = 0;
+  int op = 0;  // 0 none, 1 device CSR, 2 user MVFunc
+  DevMatrix A;
+  MVFunc func = nullptr;
+  void* func_data = nullptr;
+  hipStream_t stream = nullptr;  // the legacy null stream (serialises with user code)
+  int filter_prec = EVSL_PREC_FP64;
+  int refine_passes = 1;
+  // scratch for reductions
+  DevBuf red_part, red_scalars;
+  // pinned host scalars for device->host reads
+  double* pinned = nullptr;
+  evslAmdStats astats;
+};
+
+Context& ctx();
+int ensure_started();
+// make sure the FP32 copy of the matrix values exists
+int ensure_a32();
+// y = A x for device vectors (CSR kernel or user function)
+int apply_A(const double* x, double* y);
+// device scratch for reductions (>= kRedBlocks * 64 doubles, plus scalars)
+double* red_part();
+double* red_scalar(int i);
+// copy cnt doubles from device to the pinned host buffer and wait
+int read_scalars(const double* d, int cnt, double* out);
+
+// ---- tridiagonal eigensolver (tridiag.cpp) ----------------------------
+// Implicit QL with a Wilkinson-type shift.  Eigenvalues ascending in d.
+// If rec != nullptr the plane rotations are recorded for replay on the
+// device (the eigenvectors are then Z = I * rotations, columns permuted by
+// rec->perm to ascending order).
+struct RotationLog {
+  std::vector<int> top, cnt;
+  std::vector<long long> off;
+  std::vector<double> cs;
+  std::vector<int> perm;  // sorted position j holds original column perm[j]
+  void clear() {
+    top.clear();
+    cnt.clear();
+    off.clear();
+    cs.clear();
+    perm.clear();
+  }
+};
+int tridiag_ql(int n, double* d, double* e, RotationLog* rec);
+// eigenvalues and eigenvectors of a small symmetric tridiagonal on the host
+// (column-major n x n Z); used where T is tiny.
+int tridiag_eig_host(int n, const double* diag, const double* sdiag, double* eval,
+                     double* Z);
+// dense symmetric eigensolver on the host (Householder + QL), column-major
+int sym_eig_host(int n, const double* A, int lda, double* Q, int ldq, double* lam);
+
+// Replay a rotation log on the device: Zsel(:, j) = eigenvector of T at
+// sorted position sel[j] (kdim x m, column-major, leading dimension kdim).
+int tridiag_rotations_to_device(int kdim, const RotationLog& rec, const std::vector<int>& sel,
+                                double* Zsel_d);
+
+// ---- polynomial helpers (filters.cpp) ----------------------------------
+int dampcf(int m, int damping, double* jac);
+int chebxPltd(int m, const double* mu, int npts, const double* xi, double* yi);
+
+// ---- filter application (lanczos.cpp) -----------------------------------
+struct ChebWork {
+  DevBuf bufs;  // 3 n-vectors
+  DevBuf mu_d;  // filter coefficients on the device
+  DevBuf mu0_d;
+  DevBuf b32;   // FP32 work (4 n-vectors) for the mixed filter
+  int deg_loaded = -1;
+  const double* mu_src = nullptr;
+};
+// y = p(A) v (device vectors) with pol; FP64 or FP32 per ctx().filter_prec
+int cheb_av(const polparams* pol, const double* v, double* y, ChebWork& w, int prec);
+
+}  // namespace evsl

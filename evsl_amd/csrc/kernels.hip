@@ -1,0 +1,828 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the filtered-Lanczos / KPM path.
+//
+// Design notes (wave64, 256-thread workgroups):
+//  * cheb_spmm: CSR row blocks are staged through LDS with coalesced loads of
+//    ja/a, then each (row, column) lane walks its row in LDS and gathers the
+//    B contiguous values of x it needs.  The Chebyshev three-term recurrence
+//    (and either the ChebAv accumulation or the KPM moment dot) is fused into
+//    the epilogue, so A is read once per degree for all B columns.  Workgroup
+//    ids are remapped so that each of the 8 XCDs sweeps a contiguous range of
+//    rows (its own L2 then holds the neighbouring x segments it re-reads).
+//  * Reductions are two-level and deterministic: per-block partials, then a
+//    separate small launch sums them in a fixed order.
+//  * Tall-skinny Gram-Schmidt: V^T w by column groups of 8 with per-chunk
+//    partials; V h with h staged in LDS.  Both stream V once (HBM-bound).
+#include <hip/hip_runtime.h>
+
+#include "dev.h"
+
+namespace evsl_dev {
+namespace {
+
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+  const int xcd = bid % kXcds, pos = bid / kXcds;
+  const int per = nblk / kXcds, rem = nblk % kXcds;
+  return xcd * per + min(xcd, rem) + pos;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// sum over a 256-thread block; result valid in thread 0
+__device__ __forceinline__ double block_sum(double v, double* sh /*>=4*/) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) r = sh[0] + sh[1] + sh[2] + sh[3];
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------------
+// fused Chebyshev step SpMM
+// ---------------------------------------------------------------------
+template <typename VT, int B, int MODE>
+__global__ __launch_bounds__(kWG) void k_cheb_spmm(
+    const int* __restrict__ ia, const int* __restrict__ ja, const VT* __restrict__ av,
+    const int* __restrict__ rb, int nblk, const VT* __restrict__ xk,
+    const VT* __restrict__ xkm1, VT* __restrict__ xkp1, VT* __restrict__ y,
+    const VT* __restrict__ v0, const double* __restrict__ scol,
+    const double* __restrict__ mu0col, double t_, double cc_, int first,
+    double* __restrict__ part) {
+  __shared__ int s_ja[kNnzCap];
+  __shared__ VT s_a[kNnzCap];
+  __shared__ double s_red[kWG];
+  constexpr int NG = kWG / B;
+  const int blk = xcd_remap(blockIdx.x, nblk);
+  const int r0 = rb[blk], r1 = rb[blk + 1];
+  const int p0 = ia[r0], p1 = ia[r1];
+  const int nz = p1 - p0;
+  const int col = threadIdx.x % B, grp = threadIdx.x / B;
+  const VT t = (VT)t_, cc = (VT)cc_;
+  VT s = 0, mu0 = 0;
+  if (MODE == kCheb) {
+    s = (VT)scol[col];
+    if (first) mu0 = (VT)mu0col[col];
+  }
+  double dacc = 0.0;
+
+  auto epilogue = [&](int r, VT acc) {
+    const size_t o = (size_t)r * B + col;
+    if (MODE == kPlain) {
+      xkp1[o] = acc;
+      return;
+    }
+    const VT xr = xk[o];
+    VT w = t * (acc - cc * xr);
+    if (xkm1) w -= xkm1[o];
+    xkp1[o] = w;
+    if (MODE == kCheb) {
+      const VT yo = first ? mu0 * xr : y[o];
+      y[o] = yo + s * w;
+    } else {
+      dacc += (double)w * (double)v0[o];
+    }
+  };
+
+  if (nz <= kNnzCap) {
+    for (int p = threadIdx.x; p < nz; p += kWG) {
+      s_ja[p] = ja[p0 + p];
+      s_a[p] = av[p0 + p];
+    }
+    __syncthreads();
+    for (int r = r0 + grp; r < r1; r += NG) {
+      const int q0 = ia[r] - p0, q1 = ia[r + 1] - p0;
+      VT acc = 0;
+      for (int q = q0; q < q1; ++q) acc += s_a[q] * xk[(size_t)s_ja[q] * B + col];
+      epilogue(r, acc);
+    }
+  } else {
+    // one long row: all groups split its nonzeros, combine through LDS
+    VT acc = 0;
+    for (int p = p0 + grp; p < p1; p += NG) acc += av[p] * xk[(size_t)ja[p] * B + col];
+    s_red[threadIdx.x] = (double)acc;
+    __syncthreads();
+    for (int st = NG / 2; st > 0; st >>= 1) {
+      if (grp < st) s_red[threadIdx.x] += s_red[threadIdx.x + st * B];
+      __syncthreads();
+    }
+    if (grp == 0) epilogue(r0, (VT)s_red[col]);
+    __syncthreads();
+  }
+
+  if (MODE == kKpm) {
+    s_red[threadIdx.x] = dacc;
+    __syncthreads();
+    for (int st = NG / 2; st > 0; st >>= 1) {
+      if (grp < st) s_red[threadIdx.x] += s_red[threadIdx.x + st * B];
+      __syncthreads();
+    }
+    if (grp == 0) part[(size_t)blk * B + col] = s_red[col];
+  }
+}
+
+// out[c] = sum_b part[b*B + c]
+__global__ __launch_bounds__(kWG) void k_sum_block_partials(const double* __restrict__ part,
+                                                            int nblk, int B,
+                                                            double* __restrict__ out) {
+  __shared__ double s_red[kWG];
+  const int col = threadIdx.x % B, grp = threadIdx.x / B, NG = kWG / B;
+  double a = 0.0;
+  for (int b = grp; b < nblk; b += NG) a += part[(size_t)b * B + col];
+  s_red[threadIdx.x] = a;
+  __syncthreads();
+  for (int st = NG / 2; st > 0; st >>= 1) {
+    if (grp < st) s_red[threadIdx.x] += s_red[threadIdx.x + st * B];
+    __syncthreads();
+  }
+  if (grp == 0) out[col] = s_red[col];
+}
+
+// ---------------------------------------------------------------------
+// BLAS-1 style kernels with fixed-grid partial reductions
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(kWG) void k_dot_partial(size_t n, const double* __restrict__ x,
+                                                     const double* __restrict__ y,
+                                                     double* __restrict__ part) {
+  __shared__ double sh[4];
+  double a = 0.0;
+  for (size_t i = (size_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (size_t)gridDim.x * kWG)
+    a += x[i] * y[i];
+  a = block_sum(a, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
+__global__ __launch_bounds__(kWG) void k_sum_scalar(const double* __restrict__ part, int cnt,
+                                                    double* __restrict__ out) {
+  __shared__ double sh[4];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < cnt; i += kWG) a += part[i];
+  a = block_sum(a, sh);
+  if (threadIdx.x == 0) *out = a;
+}
+
+__global__ __launch_bounds__(kWG) void k_axpy_dot(size_t n, double a,
+                                                  const double* __restrict__ x,
+                                                  double* __restrict__ y,
+                                                  const double* __restrict__ z,
+                                                  double* __restrict__ part) {
+  __shared__ double sh[4];
+  double d = 0.0;
+  for (size_t i = (size_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (size_t)gridDim.x * kWG) {
+    const double v = y[i] + a * x[i];
+    y[i] = v;
+    if (z) d += v * z[i];
+  }
+  if (z) {
+    d = block_sum(d, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = d;
+  }
+}
+
+template <bool SELF>
+__global__ __launch_bounds__(kWG) void k_axpy_dev(size_t n, double sign,
+                                                  const double* __restrict__ a_d,
+                                                  const double* __restrict__ x,
+                                                  double* __restrict__ y,
+                                                  const double* __restrict__ z,
+                                                  double* __restrict__ part) {
+  __shared__ double sh[4];
+  const double a = sign * (*a_d);
+  double d = 0.0;
+  for (size_t i = (size_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (size_t)gridDim.x * kWG) {
+    const double v = y[i] + a * x[i];
+    y[i] = v;
+    d += v * (SELF ? v : z[i]);
+  }
+  d = block_sum(d, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = d;
+}
+
+__global__ void k_scal(size_t n, double a, double* __restrict__ x) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    x[i] *= a;
+}
+
+__global__ void k_scal_copy(size_t n, double a, const double* __restrict__ x,
+                            double* __restrict__ y) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    y[i] = a * x[i];
+}
+
+__global__ void k_fill(size_t n, double v, double* __restrict__ x) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+
+__global__ void k_set_identity(int k, double* __restrict__ Z) {
+  const size_t tot = (size_t)k * k;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
+       i += (size_t)gridDim.x * blockDim.x)
+    Z[i] = (i % (size_t)(k + 1) == 0) ? 1.0 : 0.0;
+}
+
+__global__ void k_fill_u32(size_t n, unsigned v, unsigned* __restrict__ x) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+
+__global__ void k_f64_to_f32(size_t n, const double* __restrict__ x, float* __restrict__ y) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    y[i] = (float)x[i];
+}
+
+__global__ void k_f32_to_f64(size_t n, const float* __restrict__ x, double* __restrict__ y) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    y[i] = (double)x[i];
+}
+
+// ---------------------------------------------------------------------
+// tall-skinny Gram-Schmidt kernels
+// ---------------------------------------------------------------------
+constexpr int kGtCols = 8;     // columns per workgroup in V^T w
+constexpr int kGtRows = 4096;  // rows per chunk in V^T w
+
+// part[c * nch + chunk] = sum over the chunk's rows of V(r, c) * w(r)
+__global__ __launch_bounds__(kWG) void k_gemv_t_partial(size_t n, int k,
+                                                        const double* __restrict__ V,
+                                                        size_t ld,
+                                                        const double* __restrict__ w,
+                                                        double* __restrict__ part, int nch) {
+  __shared__ double sh[4][kGtCols];
+  const int ch = blockIdx.x;
+  const int c0 = blockIdx.y * kGtCols;
+  const int nc = min(kGtCols, k - c0);
+  double acc[kGtCols];
+#pragma unroll
+  for (int c = 0; c < kGtCols; ++c) acc[c] = 0.0;
+  const size_t rbeg = (size_t)ch * kGtRows;
+  const size_t rend = min(n, rbeg + kGtRows);
+  const double* Vc = V + (size_t)c0 * ld;
+  if (nc == kGtCols) {
+    for (size_t r = rbeg + threadIdx.x; r < rend; r += kWG) {
+      const double wr = w[r];
+#pragma unroll
+      for (int c = 0; c < kGtCols; ++c) acc[c] += Vc[(size_t)c * ld + r] * wr;
+    }
+  } else {
+    for (size_t r = rbeg + threadIdx.x; r < rend; r += kWG) {
+      const double wr = w[r];
+      for (int c = 0; c < nc; ++c) acc[c] += Vc[(size_t)c * ld + r] * wr;
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < kGtCols; ++c) {
+    const double v = wave_sum(acc[c]);
+    if (lane == 0) sh[wid][c] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < nc) {
+    const int c = threadIdx.x;
+    part[(size_t)(c0 + c) * nch + ch] = sh[0][c] + sh[1][c] + sh[2][c] + sh[3][c];
+  }
+}
+
+// h[c] = sum_chunk part[c*nch + chunk]; one 64-lane wave per column
+__global__ __launch_bounds__(64) void k_gemv_t_reduce(const double* __restrict__ part,
+                                                      int nch, double* __restrict__ h) {
+  const int c = blockIdx.x;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nch; i += 64) a += part[(size_t)c * nch + i];
+  a = wave_sum(a);
+  if (threadIdx.x == 0) h[c] = a;
+}
+
+constexpr int kHTile = 2048;  // h entries staged in LDS per pass
+
+// w -= V h; part[blk] = partial ||w||^2   (SUB) or y = V h (!SUB)
+template <bool SUB>
+__global__ __launch_bounds__(kWG) void k_gemv_n(size_t n, int k, const double* __restrict__ V,
+                                                size_t ld, const double* __restrict__ h,
+                                                double* __restrict__ w,
+                                                double* __restrict__ part) {
+  __shared__ double s_h[kHTile];
+  __shared__ double sh[4];
+  const size_t r = (size_t)blockIdx.x * kWG + threadIdx.x;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  for (int cbase = 0; cbase < k; cbase += kHTile) {
+    const int nc = min(kHTile, k - cbase);
+    __syncthreads();
+    for (int c = threadIdx.x; c < nc; c += kWG) s_h[c] = h[cbase + c];
+    __syncthreads();
+    if (r < n) {
+      const double* Vr = V + (size_t)cbase * ld + r;
+      int c = 0;
+      for (; c + 4 <= nc; c += 4) {
+        a0 += Vr[(size_t)c * ld] * s_h[c];
+        a1 += Vr[(size_t)(c + 1) * ld] * s_h[c + 1];
+        a2 += Vr[(size_t)(c + 2) * ld] * s_h[c + 2];
+        a3 += Vr[(size_t)(c + 3) * ld] * s_h[c + 3];
+      }
+      for (; c < nc; ++c) a0 += Vr[(size_t)c * ld] * s_h[c];
+    }
+  }
+  const double acc = (a0 + a1) + (a2 + a3);
+  double d = 0.0;
+  if (r < n) {
+    if (SUB) {
+      const double v = w[r] - acc;
+      w[r] = v;
+      d = v * v;
+    } else {
+      w[r] = acc;
+    }
+  }
+  if (SUB) {
+    d = block_sum(d, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = d;
+  }
+}
+
+// ---------------------------------------------------------------------
+// FP64 GEMM (column-major): C = A * B, 64x64 tiles, k-step 16
+// ---------------------------------------------------------------------
+constexpr int kTM = 64, kTN = 64, kTK = 16;
+
+__global__ __launch_bounds__(kWG) void k_gemm_nn(size_t n, int m, int K,
+                                                 const double* __restrict__ A, size_t lda,
+                                                 const double* __restrict__ B, size_t ldb,
+                                                 double* __restrict__ C, size_t ldc) {
+  __shared__ double As[kTK][kTM];
+  __shared__ double Bs[kTK][kTN + 1];
+  const size_t row0 = (size_t)blockIdx.x * kTM;
+  const int col0 = blockIdx.y * kTN;
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  double c[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c[i][j] = 0.0;
+  for (int k0 = 0; k0 < K; k0 += kTK) {
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const int e = threadIdx.x + l * kWG;
+      const int rr = e % kTM, kk = e / kTM;
+      const size_t gr = row0 + rr;
+      const int gk = k0 + kk;
+      As[kk][rr] = (gr < n && gk < K) ? A[(size_t)gk * lda + gr] : 0.0;
+      const int kb = e % kTK, cb = e / kTK;
+      const int gc = col0 + cb, gkb = k0 + kb;
+      Bs[kb][cb] = (gc < m && gkb < K) ? B[(size_t)gc * ldb + gkb] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kTK; ++kk) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][tx + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][ty + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) c[i][j] += a[i] * b[j];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int gc = col0 + ty + 16 * j;
+    if (gc >= m) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const size_t gr = row0 + tx + 16 * i;
+      if (gr < n) C[(size_t)gc * ldc + gr] = c[i][j];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
+// per-column reductions (column-major), grid (chunks, m)
+// ---------------------------------------------------------------------
+constexpr int kColRows = 8192;
+
+template <int KIND>  // 0: X.Y ; 1: ||W - t U||^2
+__global__ __launch_bounds__(kWG) void k_col_partial(size_t n, const double* __restrict__ X,
+                                                     size_t ldx, const double* __restrict__ Y,
+                                                     size_t ldy, const double* __restrict__ t,
+                                                     double* __restrict__ part, int nch) {
+  __shared__ double sh[4];
+  const int ch = blockIdx.x, j = blockIdx.y;
+  const double* x = X + (size_t)j * ldx;
+  const double* y = Y + (size_t)j * ldy;
+  const double tj = KIND == 1 ? t[j] : 0.0;
+  const size_t rbeg = (size_t)ch * kColRows, rend = min(n, rbeg + kColRows);
+  double a = 0.0;
+  for (size_t r = rbeg + threadIdx.x; r < rend; r += kWG) {
+    if (KIND == 0) {
+      a += x[r] * y[r];
+    } else {
+      const double d = x[r] - tj * y[r];
+      a += d * d;
+    }
+  }
+  a = block_sum(a, sh);
+  if (threadIdx.x == 0) part[(size_t)j * nch + ch] = a;
+}
+
+__global__ void k_col_scale_rsqrt(size_t n, double* __restrict__ X, size_t ldx,
+                                  const double* __restrict__ s2) {
+  const int j = blockIdx.y;
+  const double s = s2[j] > 0.0 ? 1.0 / sqrt(s2[j]) : 0.0;
+  double* x = X + (size_t)j * ldx;
+  for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (size_t)gridDim.x * blockDim.x)
+    x[r] *= s;
+}
+
+__global__ void k_gather_cols(size_t nrow, const double* __restrict__ src, size_t lds,
+                              const int* __restrict__ idx, double* __restrict__ dst,
+                              size_t ldd) {
+  const int j = blockIdx.y;
+  const double* s = src + (size_t)idx[j] * lds;
+  double* d = dst + (size_t)j * ldd;
+  for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrow;
+       r += (size_t)gridDim.x * blockDim.x)
+    d[r] = s[r];
+}
+
+// ---------------------------------------------------------------------
+// QL rotation application: thread per row of Z, one sweep at a time; the
+// rotated pair shares column i with the next rotation, so that value is
+// carried in a register and each rotation costs one load and one store.
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(kWG) void k_apply_rotations(int k, double* __restrict__ Z,
+                                                         int nsweep,
+                                                         const int* __restrict__ top,
+                                                         const int* __restrict__ cnt,
+                                                         const long long* __restrict__ off,
+                                                         const double* __restrict__ cs) {
+  const int r = blockIdx.x * kWG + threadIdx.x;
+  if (r >= k) return;
+  double* z = Z + r;
+  const size_t ldz = (size_t)k;
+  for (int sw = 0; sw < nsweep; ++sw) {
+    const int ihi = top[sw], c = cnt[sw];
+    const double* rot = cs + 2 * off[sw];
+    double carry = z[(size_t)(ihi + 1) * ldz];
+    int j = 0;
+    for (; j + 4 <= c; j += 4) {
+      const int i = ihi - j;
+      const double z0 = z[(size_t)i * ldz];
+      const double z1 = z[(size_t)(i - 1) * ldz];
+      const double z2 = z[(size_t)(i - 2) * ldz];
+      const double z3 = z[(size_t)(i - 3) * ldz];
+      double cc = rot[2 * j], ss = rot[2 * j + 1];
+      z[(size_t)(i + 1) * ldz] = ss * z0 + cc * carry;
+      carry = cc * z0 - ss * carry;
+      cc = rot[2 * j + 2];
+      ss = rot[2 * j + 3];
+      z[(size_t)i * ldz] = ss * z1 + cc * carry;
+      carry = cc * z1 - ss * carry;
+      cc = rot[2 * j + 4];
+      ss = rot[2 * j + 5];
+      z[(size_t)(i - 1) * ldz] = ss * z2 + cc * carry;
+      carry = cc * z2 - ss * carry;
+      cc = rot[2 * j + 6];
+      ss = rot[2 * j + 7];
+      z[(size_t)(i - 2) * ldz] = ss * z3 + cc * carry;
+      carry = cc * z3 - ss * carry;
+    }
+    for (; j < c; ++j) {
+      const int i = ihi - j;
+      const double zi = z[(size_t)i * ldz];
+      const double cc = rot[2 * j], ss = rot[2 * j + 1];
+      z[(size_t)(i + 1) * ldz] = ss * zi + cc * carry;
+      carry = cc * zi - ss * carry;
+    }
+    z[(size_t)(ihi - c + 1) * ldz] = carry;
+  }
+}
+
+// ---------------------------------------------------------------------
+// row-major block helpers
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(kWG) void k_rm_col_nrm2(size_t n, int B,
+                                                     const double* __restrict__ X,
+                                                     double* __restrict__ part) {
+  __shared__ double s_red[kWG];
+  const int col = threadIdx.x % B, grp = threadIdx.x / B, NG = kWG / B;
+  double a = 0.0;
+  for (size_t r = (size_t)blockIdx.x * NG + grp; r < n; r += (size_t)gridDim.x * NG) {
+    const double v = X[r * B + col];
+    a += v * v;
+  }
+  s_red[threadIdx.x] = a;
+  __syncthreads();
+  for (int st = NG / 2; st > 0; st >>= 1) {
+    if (grp < st) s_red[threadIdx.x] += s_red[threadIdx.x + st * B];
+    __syncthreads();
+  }
+  if (grp == 0) part[(size_t)blockIdx.x * B + col] = s_red[col];
+}
+
+__global__ void k_rm_col_scale_rsqrt(size_t n, int B, double* __restrict__ X,
+                                     const double* __restrict__ s2) {
+  const size_t tot = n * (size_t)B;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double s = s2[i % B];
+    X[i] = s > 0.0 ? X[i] / sqrt(s) : 0.0;
+  }
+}
+
+// counter-based generator: splitmix64 finaliser of (seed, column, row)
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+__global__ void k_rm_random(size_t n, int B, int kind, unsigned long long seed, int col0,
+                            int ncols, double* __restrict__ X) {
+  const size_t tot = n * (size_t)B;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % B);
+    const size_t r = i / B;
+    double v = 0.0;
+    if (c < ncols) {
+      const unsigned long long key =
+          mix64(seed + 0x9e3779b97f4a7c15ULL * (unsigned long long)(col0 + c + 1)) ^ (r * 0xd1b54a32d192ed03ULL);
+      const unsigned long long h = mix64(key);
+      if (kind == 0) {  // Rademacher
+        v = (h >> 63) ? 1.0 : -1.0;
+      } else {          // Gaussian, Box-Muller on two 53-bit uniforms
+        const unsigned long long h2 = mix64(key ^ 0x632be59bd9b4e019ULL);
+        const double u1 = ((h >> 11) + 1) * (1.0 / 9007199254740993.0);
+        const double u2 = (h2 >> 11) * (1.0 / 9007199254740992.0);
+        v = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+      }
+    }
+    X[i] = v;
+  }
+}
+
+inline unsigned grid_for(size_t n, int per_block, unsigned cap) {
+  size_t g = (n + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+template <typename VT, int MODE>
+hipError_t launch_cheb_B(const Csr& A, int B, const void* xk, const void* xkm1, void* xkp1,
+                         void* y, const void* v0, const double* s, const double* mu0,
+                         double t, double cc, bool first, double* part, hipStream_t st) {
+  const VT* av;
+  if constexpr (sizeof(VT) == 4)
+    av = A.a32;
+  else
+    av = A.a64;
+  if (!av) return hipErrorInvalidValue;
+  dim3 grid(A.nblk), block(kWG);
+#define EVSL_CHEB_CASE(BB)                                                                   \
+  case BB:                                                                                   \
+    hipLaunchKernelGGL((k_cheb_spmm<VT, BB, MODE>), grid, block, 0, st, A.ia, A.ja, av,      \
+                       A.rb, A.nblk, (const VT*)xk, (const VT*)xkm1, (VT*)xkp1, (VT*)y,      \
+                       (const VT*)v0, s, mu0, t, cc, first ? 1 : 0, part);                   \
+    break;
+  switch (B) {
+    EVSL_CHEB_CASE(1)
+    EVSL_CHEB_CASE(2)
+    EVSL_CHEB_CASE(4)
+    EVSL_CHEB_CASE(8)
+    EVSL_CHEB_CASE(16)
+    EVSL_CHEB_CASE(32)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef EVSL_CHEB_CASE
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------
+hipError_t cheb_spmm(const Csr& A, int B, bool fp32, Mode mode, const void* xk,
+                     const void* xkm1, void* xkp1, void* y, const void* v0, const double* s,
+                     const double* mu0, double t, double cc, bool first, double* part,
+                     hipStream_t st) {
+  if (A.nblk <= 0) return hipSuccess;
+  if (fp32) {
+    switch (mode) {
+      case kPlain:
+        return launch_cheb_B<float, kPlain>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
+      case kCheb:
+        return launch_cheb_B<float, kCheb>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
+      case kKpm:
+        return launch_cheb_B<float, kKpm>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
+    }
+  } else {
+    switch (mode) {
+      case kPlain:
+        return launch_cheb_B<double, kPlain>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
+      case kCheb:
+        return launch_cheb_B<double, kCheb>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
+      case kKpm:
+        return launch_cheb_B<double, kKpm>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t sum_block_partials(const double* part, int nblk, int B, double* out,
+                              hipStream_t st) {
+  if (B < 1 || B > kWG || (kWG % B) != 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_sum_block_partials, dim3(1), dim3(kWG), 0, st, part, nblk, B, out);
+  return hipGetLastError();
+}
+
+hipError_t dot_partial(size_t n, const double* x, const double* y, double* part,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(kWG), 0, st, n, x, y, part);
+  return hipGetLastError();
+}
+
+hipError_t sum_scalar(const double* part, int cnt, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_sum_scalar, dim3(1), dim3(kWG), 0, st, part, cnt, out);
+  return hipGetLastError();
+}
+
+hipError_t axpy_dot(size_t n, double a, const double* x, double* y, const double* z,
+                    double* part, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy_dot, dim3(kRedBlocks), dim3(kWG), 0, st, n, a, x, y, z, part);
+  return hipGetLastError();
+}
+
+hipError_t axpy_dev_nrm(size_t n, double sign, const double* a_d, const double* x,
+                        double* y, double* part, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy_dev<true>, dim3(kRedBlocks), dim3(kWG), 0, st, n, sign, a_d, x,
+                     y, (const double*)nullptr, part);
+  return hipGetLastError();
+}
+
+hipError_t axpy_dev_dot(size_t n, double sign, const double* a_d, const double* x,
+                        double* y, const double* z, double* part, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy_dev<false>, dim3(kRedBlocks), dim3(kWG), 0, st, n, sign, a_d, x,
+                     y, z, part);
+  return hipGetLastError();
+}
+
+hipError_t scal(size_t n, double a, double* x, hipStream_t st) {
+  hipLaunchKernelGGL(k_scal, dim3(grid_for(n, kWG, 4096)), dim3(kWG), 0, st, n, a, x);
+  return hipGetLastError();
+}
+
+hipError_t scal_copy(size_t n, double a, const double* x, double* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_scal_copy, dim3(grid_for(n, kWG, 4096)), dim3(kWG), 0, st, n, a, x, y);
+  return hipGetLastError();
+}
+
+hipError_t fill(size_t n, double v, double* x, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n, kWG, 4096)), dim3(kWG), 0, st, n, v, x);
+  return hipGetLastError();
+}
+
+hipError_t fill_u32(size_t n, unsigned v, unsigned* x, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, kWG, 4096)), dim3(kWG), 0, st, n, v, x);
+  return hipGetLastError();
+}
+
+hipError_t set_identity(int k, double* Z, hipStream_t st) {
+  const size_t tot = (size_t)k * k;
+  hipLaunchKernelGGL(k_set_identity, dim3(grid_for(tot, kWG, 4096)), dim3(kWG), 0, st, k, Z);
+  return hipGetLastError();
+}
+
+hipError_t f64_to_f32(size_t n, const double* x, float* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(n, kWG, 4096)), dim3(kWG), 0, st, n, x, y);
+  return hipGetLastError();
+}
+
+hipError_t f32_to_f64(size_t n, const float* x, double* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_f32_to_f64, dim3(grid_for(n, kWG, 4096)), dim3(kWG), 0, st, n, x, y);
+  return hipGetLastError();
+}
+
+int gemv_t_chunks(size_t n) { return (int)((n + kGtRows - 1) / kGtRows); }
+
+hipError_t gemv_t(size_t n, int k, const double* V, size_t ld, const double* w,
+                  double* part, double* h, hipStream_t st) {
+  if (k <= 0) return hipSuccess;
+  const int nch = gemv_t_chunks(n);
+  dim3 grid(nch, (k + kGtCols - 1) / kGtCols);
+  hipLaunchKernelGGL(k_gemv_t_partial, grid, dim3(kWG), 0, st, n, k, V, ld, w, part, nch);
+  hipLaunchKernelGGL(k_gemv_t_reduce, dim3(k), dim3(64), 0, st, part, nch, h);
+  return hipGetLastError();
+}
+
+hipError_t gemv_n_sub_nrm(size_t n, int k, const double* V, size_t ld, const double* h,
+                          double* w, double* part, hipStream_t st) {
+  const unsigned g = (unsigned)((n + kWG - 1) / kWG);
+  hipLaunchKernelGGL(k_gemv_n<true>, dim3(g), dim3(kWG), 0, st, n, k, V, ld, h, w, part);
+  return hipGetLastError();
+}
+
+hipError_t gemv_n(size_t n, int k, const double* V, size_t ld, const double* h, double* y,
+                  hipStream_t st) {
+  const unsigned g = (unsigned)((n + kWG - 1) / kWG);
+  hipLaunchKernelGGL(k_gemv_n<false>, dim3(g), dim3(kWG), 0, st, n, k, V, ld, h, y,
+                     (double*)nullptr);
+  return hipGetLastError();
+}
+
+hipError_t gemm_nn(size_t n, int m, int k, const double* A, size_t lda, const double* B,
+                   size_t ldb, double* C, size_t ldc, hipStream_t st) {
+  if (n == 0 || m <= 0) return hipSuccess;
+  dim3 grid((unsigned)((n + kTM - 1) / kTM), (m + kTN - 1) / kTN);
+  hipLaunchKernelGGL(k_gemm_nn, grid, dim3(kWG), 0, st, n, m, k, A, lda, B, ldb, C, ldc);
+  return hipGetLastError();
+}
+
+static int col_chunks(size_t n) { return (int)((n + kColRows - 1) / kColRows); }
+
+hipError_t col_dot(size_t n, int m, const double* X, size_t ldx, const double* Y, size_t ldy,
+                   double* part, double* out, hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  const int nch = col_chunks(n);
+  hipLaunchKernelGGL(k_col_partial<0>, dim3(nch, m), dim3(kWG), 0, st, n, X, ldx, Y, ldy,
+                     (const double*)nullptr, part, nch);
+  hipLaunchKernelGGL(k_gemv_t_reduce, dim3(m), dim3(64), 0, st, part, nch, out);
+  return hipGetLastError();
+}
+
+hipError_t col_res2(size_t n, int m, const double* W, size_t ldw, const double* U, size_t ldu,
+                    const double* t, double* part, double* out, hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  const int nch = col_chunks(n);
+  hipLaunchKernelGGL(k_col_partial<1>, dim3(nch, m), dim3(kWG), 0, st, n, W, ldw, U, ldu, t,
+                     part, nch);
+  hipLaunchKernelGGL(k_gemv_t_reduce, dim3(m), dim3(64), 0, st, part, nch, out);
+  return hipGetLastError();
+}
+
+hipError_t col_scale_rsqrt(size_t n, int m, double* X, size_t ldx, const double* s2,
+                           hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_col_scale_rsqrt, dim3(grid_for(n, kWG, 256), m), dim3(kWG), 0, st, n,
+                     X, ldx, s2);
+  return hipGetLastError();
+}
+
+hipError_t gather_cols(size_t nrow, int m, const double* src, size_t lds, const int* idx,
+                       double* dst, size_t ldd, hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_cols, dim3(grid_for(nrow, kWG, 256), m), dim3(kWG), 0, st,
+                     nrow, src, lds, idx, dst, ldd);
+  return hipGetLastError();
+}
+
+hipError_t apply_rotations(int k, double* Z, int nsweep, const int* top, const int* cnt,
+                           const long long* off, const double* cs, hipStream_t st) {
+  if (k <= 0 || nsweep <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_apply_rotations, dim3((k + kWG - 1) / kWG), dim3(kWG), 0, st, k, Z,
+                     nsweep, top, cnt, off, cs);
+  return hipGetLastError();
+}
+
+hipError_t rm_col_nrm2(size_t n, int B, const double* X, double* part, double* out,
+                       hipStream_t st) {
+  if (B < 1 || B > kWG || (kWG % B) != 0) return hipErrorInvalidValue;
+  const int NG = kWG / B;
+  const unsigned g = grid_for(n, NG, 1024);
+  hipLaunchKernelGGL(k_rm_col_nrm2, dim3(g), dim3(kWG), 0, st, n, B, X, part);
+  hipLaunchKernelGGL(k_sum_block_partials, dim3(1), dim3(kWG), 0, st, part, (int)g, B, out);
+  return hipGetLastError();
+}
+
+hipError_t rm_col_scale_rsqrt(size_t n, int B, double* X, const double* s2, hipStream_t st) {
+  hipLaunchKernelGGL(k_rm_col_scale_rsqrt, dim3(grid_for(n * B, kWG, 8192)), dim3(kWG), 0, st,
+                     n, B, X, s2);
+  return hipGetLastError();
+}
+
+hipError_t rm_random(size_t n, int B, int kind, unsigned long long seed, int col0, int ncols,
+                     double* X, hipStream_t st) {
+  hipLaunchKernelGGL(k_rm_random, dim3(grid_for(n * B, kWG, 8192)), dim3(kWG), 0, st, n, B,
+                     kind, seed, col0, ncols, X);
+  return hipGetLastError();
+}
+
+}  // namespace evsl_dev

@@ -1,0 +1,281 @@
+// kpm.cpp -- KPM density of states on the device and the spectrum slicers.
+//
+// kpmdos follows SRC/spslice.c:29-148 of EVSL 1.1.1: Chebyshev moments
+// mu_k = (2 - delta_k0) jac_k / (nvec pi) * sum_m v_m^T T_k(A~) v_m and the
+// eigenvalue count from their exact integral.  The nvec random vectors are
+// independent, so they run as column blocks of up to 32: one fused SpMM
+// launch per degree advances the three-term recurrence of every column and
+// emits the per-column dot products v_m^T T_k v_m, reading A once per degree
+// for the whole block.  spslicer/intChx (SRC/spslice.c:155-297) and
+// spslicer2 (SRC/spslice2.c:25-49) are host scalar code as in the reference.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+
+namespace evsl {
+namespace {
+
+int kpm_impl(int Mdeg, int damping, int nvec, const double* intv, int kind,
+             unsigned long long seed, double* mu, double* ecnt) {
+  if (int e = ensure_started()) return e;
+  Context& c = ctx();
+  if (c.op == 0) return EVSL_ERR_NOMAT;
+  if (intv[0] >= intv[1] || intv[0] >= intv[3] || intv[1] <= intv[2]) return -1;
+  if (Mdeg < 0 || nvec <= 0) return EVSL_ERR_ARG;
+  const int n = c.n;
+  const double aa = std::max(intv[0], intv[2]), bb = std::min(intv[1], intv[3]);
+  const double ctr = (intv[3] + intv[2]) / 2.0, wid = (intv[3] - intv[2]) / 2.0;
+  const double beta1 = std::acos(std::max(-1.0 + DBL_EPSILON, (aa - ctr) / wid));
+  const double beta2 = std::acos(std::min(1.0 - DBL_EPSILON, (bb - ctr) / wid));
+  std::vector<double> jac(Mdeg + 1);
+  dampcf(Mdeg, damping, jac.data());
+  jac[0] = 1.0;
+  std::vector<double> raw(Mdeg + 1, 0.0);  // raw[k] = sum_m v_m . T_k v_m
+  const int Bmax = 32;
+  DevBuf X0, X, dots;
+  EVSL_HIP(X0.alloc((size_t)n * Bmax * sizeof(double)));
+  EVSL_HIP(X.alloc((size_t)3 * n * Bmax * sizeof(double)));
+  EVSL_HIP(dots.alloc((size_t)(Mdeg + 1) * Bmax * sizeof(double)));
+  std::vector<double> hbuf, hcol, hdots;
+  const evsl_dev::Csr A = c.A.view();
+  int done = 0;
+  while (done < nvec) {
+    int B = Bmax;
+    while (B > nvec - done) B >>= 1;
+    const size_t nb = (size_t)n * B;
+    double* x0 = X0.as<double>();
+    if (kind == EVSL_VEC_RAND_DOUBLE) {
+      // the libc stream, vector after vector, as the reference draws it
+      hbuf.resize(nb);
+      hcol.resize(n);
+      for (int col = 0; col < B; ++col) {
+        rand_double(n, hcol.data());
+        for (int r = 0; r < n; ++r) hbuf[(size_t)r * B + col] = hcol[r];
+      }
+      EVSL_HIP(hipMemcpy(x0, hbuf.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+      EVSL_HIP(evsl_dev::rm_random(n, B, kind == EVSL_VEC_RADEMACHER ? 0 : 1, seed, done, B, x0,
+                                   c.stream));
+    }
+    double* s2 = red_scalar(8);
+    EVSL_HIP(evsl_dev::rm_col_nrm2(n, B, x0, red_part(), s2, c.stream));
+    EVSL_HIP(evsl_dev::rm_col_scale_rsqrt(n, B, x0, s2, c.stream));
+    double* buf[3] = {X.as<double>(), X.as<double>() + nb, X.as<double>() + 2 * nb};
+    if (c.op == 1) {
+      for (int k = 0; k < Mdeg; ++k) {
+        const double sc = (k == 0 ? 1.0 : 2.0) / wid;
+        void* xkp1 = buf[k % 3];
+        const void* xk = (k == 0) ? x0 : buf[(k - 1) % 3];
+        const void* xkm1 = (k == 0) ? nullptr : (k == 1 ? x0 : buf[(k - 2) % 3]);
+        EVSL_HIP(evsl_dev::cheb_spmm(A, B, false, evsl_dev::kKpm, xk, xkm1, xkp1, nullptr, x0,
+                                     nullptr, nullptr, sc, ctr, false, red_part(), c.stream));
+        EVSL_HIP(evsl_dev::sum_block_partials(red_part(), A.nblk, B,
+                                              dots.as<double>() + (size_t)(k + 1) * B,
+                                              c.stream));
+        c.astats.filter_bytes += (double)c.A.nnz * 12 + 4.0 * (n + 1) +
+                                 (double)(k == 0 ? 3 : 4) * nb * sizeof(double);
+      }
+      c.astats.n_filter_steps += Mdeg;
+      c.astats.n_filter_cols += (size_t)Mdeg * B;
+    } else {
+      // user operator: one column at a time through the callback
+      DevBuf col;
+      EVSL_HIP(col.alloc((size_t)4 * n * sizeof(double)));
+      double* v = col.as<double>();
+      double* cb[3] = {v + n, v + 2 * (size_t)n, v + 3 * (size_t)n};
+      std::vector<double> vh(nb), dh((size_t)(Mdeg + 1) * B, 0.0);
+      EVSL_HIP(hipMemcpy(vh.data(), x0, nb * sizeof(double), hipMemcpyDeviceToHost));
+      hcol.resize(n);
+      for (int cc = 0; cc < B; ++cc) {
+        for (int r = 0; r < n; ++r) hcol[r] = vh[(size_t)r * B + cc];
+        EVSL_HIP(hipMemcpy(v, hcol.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        for (int k = 0; k < Mdeg; ++k) {
+          const double sc = (k == 0 ? 1.0 : 2.0) / wid;
+          double* xkp1 = cb[k % 3];
+          const double* xk = (k == 0) ? v : cb[(k - 1) % 3];
+          const double* xkm1 = (k == 0) ? nullptr : (k == 1 ? v : cb[(k - 2) % 3]);
+          if (int e = apply_A(xk, xkp1)) return e;
+          EVSL_HIP(evsl_dev::scal(n, sc, xkp1, c.stream));
+          EVSL_HIP(evsl_dev::axpy_dot(n, -sc * ctr, xk, xkp1, nullptr, nullptr, c.stream));
+          if (xkm1) EVSL_HIP(evsl_dev::axpy_dot(n, -1.0, xkm1, xkp1, nullptr, nullptr, c.stream));
+          EVSL_HIP(evsl_dev::dot_partial(n, xkp1, v, red_part(), c.stream));
+          EVSL_HIP(evsl_dev::sum_scalar(red_part(), evsl_dev::kRedBlocks, red_scalar(0),
+                                        c.stream));
+          double d;
+          if (int e = read_scalars(red_scalar(0), 1, &d)) return e;
+          dh[(size_t)(k + 1) * B + cc] = d;
+        }
+      }
+      EVSL_HIP(hipMemcpy(dots.p, dh.data(), dh.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    hdots.resize((size_t)(Mdeg + 1) * B);
+    EVSL_HIP(hipMemcpy(hdots.data(), dots.p, hdots.size() * sizeof(double),
+                       hipMemcpyDeviceToHost));
+    for (int k = 1; k <= Mdeg; ++k)
+      for (int col = 0; col < B; ++col) raw[k] += hdots[(size_t)k * B + col];
+    done += B;
+  }
+  double tcnt = -(double)nvec * jac[0] * (beta2 - beta1);
+  mu[0] = (double)nvec * jac[0];
+  for (int k = 1; k <= Mdeg; ++k) {
+    const double t = 2 * jac[k] * raw[k];
+    mu[k] = t;
+    tcnt -= t * (std::sin(k * beta2) - std::sin(k * beta1)) / k;
+  }
+  const double t = 1.0 / (((double)nvec) * EVSL_PI);
+  for (int k = 0; k <= Mdeg; ++k) mu[k] *= t;
+  *ecnt = tcnt * t * (double)n;
+  return 0;
+}
+
+// yi[j] = integral_{xi[0]}^{xi[j]} of the KPM density (SRC/spslice.c:155-181)
+void int_chx(int Mdeg, const double* mu, int npts, const double* xi, double* yi) {
+  std::vector<double> th(npts);
+  for (int j = 0; j < npts; j++) th[j] = std::acos(xi[j]);
+  const double th0 = th[0];
+  for (int j = 0; j < npts; j++) yi[j] = mu[0] * (th0 - th[j]);
+  for (int k = 1; k <= Mdeg; k++) {
+    const double v0 = std::sin(k * th0) / k;
+    for (int j = 0; j < npts; j++) yi[j] += mu[k] * (v0 - std::sin(k * th[j]) / k);
+  }
+}
+
+}  // namespace
+}  // namespace evsl
+
+using namespace evsl;
+
+extern "C" {
+
+int kpmdos(int Mdeg, int damping, int nvec, double* intv, double* mu, double* ecnt) {
+  return kpm_impl(Mdeg, damping, nvec, intv, EVSL_VEC_RAND_DOUBLE, 0ULL, mu, ecnt);
+}
+
+int kpmdos_ex(int Mdeg, int damping, int nvec, double* intv, int vec_kind,
+              unsigned long long seed, double* mu, double* ecnt) {
+  if (vec_kind < EVSL_VEC_RAND_DOUBLE || vec_kind > EVSL_VEC_GAUSSIAN) return EVSL_ERR_ARG;
+  return kpm_impl(Mdeg, damping, nvec, intv, vec_kind, seed, mu, ecnt);
+}
+
+// equal-count slices on a linspace grid (SRC/spslice.c:205-297)
+int spslicer(double* sli, double* mu, int Mdeg, double* intv, int n_int, int npts) {
+  if (intv[0] >= intv[1] || intv[0] >= intv[3] || intv[1] <= intv[2]) return -1;
+  const double aa = std::max(intv[0], intv[2]), bb = std::min(intv[1], intv[3]);
+  std::memset(sli, 0, (size_t)(n_int + 1) * sizeof(double));
+  if (n_int == 1) {
+    sli[0] = intv[0];
+    sli[1] = intv[1];
+    return 0;
+  }
+  const double ctr = (intv[3] + intv[2]) / 2, wid = (intv[3] - intv[2]) / 2;
+  const double aL = std::max((aa - ctr) / wid, -1.0), bL = std::min((bb - ctr) / wid, 1.0);
+  npts = std::max(npts, 2 * n_int + 1);
+  std::vector<double> xi(npts), yi(npts);
+  linspace(aL, bL, npts, xi.data());
+  int_chx(Mdeg, mu, npts, xi.data(), yi.data());
+  double target = yi[npts - 1] / (double)n_int;
+  int ls = 0, ii = 0, err = 0;
+  sli[0] = intv[0];
+  while (++ls < n_int) {
+    while (ii < npts && yi[ii] < target) ii++;
+    if (ii == npts) break;
+    if ((target - yi[ii - 1]) < (yi[ii] - target)) ii--;
+    sli[ls] = ctr + wid * xi[ii];
+    target = yi[ii] + (yi[npts - 1] - yi[ii]) / (n_int - ls);
+  }
+  sli[n_int] = intv[1];
+  if (ls != n_int) err = 1;
+  for (ii = 1; ii <= n_int; ii++)
+    if (sli[ii] < sli[ii - 1]) {
+      err += 2;
+      break;
+    }
+  return err;
+}
+
+// slices from a Lanczos DOS curve: Simpson integral, then equal shares
+// (SRC/spslice2.c:25-49, SRC/simpson.c:25-50)
+void spslicer2(double* xi, double* yi, int n_int, int npts, double* sli) {
+  double tc = 0.0, ti, tm, ysav = yi[0];
+  int i = 1;
+  yi[0] = 0;
+  while (i < npts - 1) {
+    ti = (xi[i] - xi[i - 1]) * (ysav + yi[i]) / 2.0;
+    tm = tc + ti;
+    ti = (xi[i + 1] - xi[i - 1]) * (ysav + 4 * yi[i] + yi[i + 1]) / 6.0;
+    tc += ti;
+    yi[i] = tm;
+    ysav = yi[i + 1];
+    yi[i + 1] = tc;
+    i += 2;
+  }
+  if (i == npts - 1) yi[i] = tc + (xi[i] - xi[i - 1]) * (yi[i] + ysav) / 2.0;
+  double t = yi[0];
+  const double want = (yi[npts - 1] - yi[0]) / (double)n_int;
+  int ls = 0;
+  sli[ls] = xi[0];
+  for (int k = 1; k < npts; k++) {
+    if (yi[k] - t >= want) {
+      ls = ls + 1;
+      sli[ls] = xi[k];
+      t = yi[k];
+    }
+  }
+  sli[n_int] = xi[npts - 1];
+}
+
+// Time nsteps fused filter launches (block width B, prec) on the current
+// matrix with HIP events; reports the average launch time and the
+// algorithmic bytes of one launch.  mode 1: ChebAv step, 2: KPM step.
+int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double* ms_per_step,
+                          double* bytes_per_step) {
+  if (int e = ensure_started()) return e;
+  Context& c = ctx();
+  if (c.op != 1) return EVSL_ERR_NOMAT;
+  if (nsteps <= 0) return EVSL_ERR_ARG;
+  const bool f32 = (prec == EVSL_PREC_MIXED);
+  if (f32)
+    if (int e = ensure_a32()) return e;
+  const int n = c.n;
+  const size_t vs = f32 ? 4 : 8;
+  const size_t nb = (size_t)n * B;
+  DevBuf X, coef;
+  EVSL_HIP(X.alloc(5 * nb * vs));
+  EVSL_HIP(coef.alloc(2 * 64 * sizeof(double)));
+  char* base = X.as<char>();
+  EVSL_HIP(hipMemsetAsync(base, 0, 5 * nb * vs, c.stream));
+  EVSL_HIP(evsl_dev::fill(2 * 64, 0.01, coef.as<double>(), c.stream));
+  void* buf[5];
+  for (int i = 0; i < 5; ++i) buf[i] = base + i * nb * vs;
+  const evsl_dev::Mode md = (mode == 2) ? evsl_dev::kKpm : evsl_dev::kCheb;
+  const evsl_dev::Csr A = c.A.view();
+  hipEvent_t e0, e1;
+  EVSL_HIP(hipEventCreate(&e0));
+  EVSL_HIP(hipEventCreate(&e1));
+  for (int it = 0; it < nsteps + 2; ++it) {
+    if (it == 2) EVSL_HIP(hipEventRecord(e0, c.stream));
+    EVSL_HIP(evsl_dev::cheb_spmm(A, B, f32, md, buf[it % 3], buf[(it + 1) % 3],
+                                 buf[(it + 2) % 3], buf[3], buf[4], coef.as<double>(),
+                                 coef.as<double>() + 64, 0.1, 0.5, false, red_part(),
+                                 c.stream));
+  }
+  EVSL_HIP(hipEventRecord(e1, c.stream));
+  EVSL_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  EVSL_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_per_step = ms / nsteps;
+  const int streams = (mode == 2) ? 4 : 5;
+  *bytes_per_step = (double)c.A.nnz * (vs + 4) + 4.0 * (n + 1) + (double)streams * nb * vs;
+  return 0;
+}
+
+}  // extern "C"

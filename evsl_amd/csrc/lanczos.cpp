@@ -1,0 +1,605 @@
+// lanczos.cpp -- filtered Lanczos on the device: ChebAv, CGS/DGKS,
+// ChebLanNr, LanBounds and LanDos of the drop-in interface.
+//
+// Follows SRC/chebpoly.c:519-598 (ChebAv), SRC/misc_la.c:221-261 (CGS_DGKS),
+// SRC/cheblanNr.c:57-434, SRC/lanbounds.c:21-144 and SRC/landos.c:37-196 of
+// EVSL 1.1.1.  The Lanczos basis V stays in HBM (column-major, n x (maxit+1));
+// all vector work runs in HIP kernels; the host keeps the scalar recurrence
+// (alpha, beta), the tridiagonal eigenvalues and the control flow.  One
+// device->host read per Lanczos step carries alpha and the DGKS norms.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+
+namespace evsl {
+
+namespace {
+
+constexpr double kOrthTol = 1e-14;  // INC/internal_header.h:21
+constexpr int kNgsMax = 2;          // INC/internal_header.h:24
+const double kEpsM = EVSL_DBL_EPS_MULT * DBL_EPSILON;
+
+int check_intv(const double* intv) {
+  if (intv[0] >= intv[1]) return -1;
+  if (intv[0] >= intv[3] || intv[1] <= intv[2]) return -2;
+  return 0;
+}
+
+// device pointer for a caller vector that may live on the host or device
+int as_device_vector(const double* p, int n, DevBuf& tmp, const double** out) {
+  hipPointerAttribute_t at;
+  hipError_t e = hipPointerGetAttributes(&at, p);
+  if (e == hipSuccess && (at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged)) {
+    *out = p;
+    return 0;
+  }
+  (void)hipGetLastError();  // clear the error of an unregistered host pointer
+  EVSL_HIP(tmp.alloc((size_t)n * sizeof(double)));
+  EVSL_HIP(hipMemcpy(tmp.p, p, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  *out = tmp.as<double>();
+  return 0;
+}
+
+// HIP-event phase timing accumulated into the extension statistics
+struct EventClock {
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool ok = false;
+  EventClock() {
+    ok = true;
+    for (auto& e : ev)
+      if (hipEventCreate(&e) != hipSuccess) ok = false;
+  }
+  ~EventClock() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  void mark(int i) {
+    if (ok) (void)hipEventRecord(ev[i], ctx().stream);
+  }
+  double secs(int a, int b) {
+    float ms = 0.f;
+    if (!ok || hipEventSynchronize(ev[b]) != hipSuccess ||
+        hipEventElapsedTime(&ms, ev[a], ev[b]) != hipSuccess)
+      return 0.0;
+    return ms * 1e-3;
+  }
+};
+
+double filter_step_bytes(const Context& c, int B, int sv, int sa, int streams) {
+  return (double)c.A.nnz * (sa + 4) + 4.0 * (c.n + 1) + (double)streams * B * c.n * sv;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// y = p((A - cc)/dd) v  -- one fused SpMM launch per degree
+// ---------------------------------------------------------------------
+int cheb_av(const polparams* pol, const double* v, double* y, ChebWork& w, int prec) {
+  Context& c = ctx();
+  const int n = c.n, m = pol->deg;
+  const double cc = pol->cc, t1 = 1.0 / pol->dd, t2 = 2.0 / pol->dd;
+  if (w.deg_loaded != m || w.mu_src != pol->mu) {
+    EVSL_HIP(w.mu_d.alloc((size_t)(m + 1) * sizeof(double)));
+    EVSL_HIP(hipMemcpy(w.mu_d.p, pol->mu, (size_t)(m + 1) * sizeof(double),
+                       hipMemcpyHostToDevice));
+    w.deg_loaded = m;
+    w.mu_src = pol->mu;
+  }
+  const double* mu_d = w.mu_d.as<double>();
+  if (m == 0) {
+    EVSL_HIP(evsl_dev::scal_copy(n, pol->mu[0], v, y, c.stream));
+    return 0;
+  }
+  if (c.op == 2 && prec != EVSL_PREC_FP64) prec = EVSL_PREC_FP64;  // user operator: FP64 only
+  if (c.op == 1) {
+    const bool f32 = (prec == EVSL_PREC_MIXED);
+    const size_t vs = f32 ? sizeof(float) : sizeof(double);
+    if (f32) EVSL_HIP(ensure_a32() == 0 ? hipSuccess : hipErrorInvalidValue);
+    DevBuf& store = f32 ? w.b32 : w.bufs;
+    const int nvec = f32 ? 5 : 3;
+    EVSL_HIP(store.alloc((size_t)nvec * n * vs));
+    char* base = store.as<char>();
+    void* buf[3] = {base, base + (size_t)n * vs, base + 2 * (size_t)n * vs};
+    const void* vin = v;
+    void* yout = y;
+    if (f32) {
+      void* v32 = base + 3 * (size_t)n * vs;
+      yout = base + 4 * (size_t)n * vs;
+      EVSL_HIP(evsl_dev::f64_to_f32(n, v, (float*)v32, c.stream));
+      vin = v32;
+    }
+    const evsl_dev::Csr A = c.A.view();
+    for (int k = 1; k <= m; ++k) {
+      void* xkp1 = buf[(k - 1) % 3];
+      const void* xk = (k == 1) ? vin : buf[(k - 2) % 3];
+      const void* xkm1 = (k == 1) ? nullptr : (k == 2 ? vin : buf[(k - 3) % 3]);
+      EVSL_HIP(evsl_dev::cheb_spmm(A, 1, f32, evsl_dev::kCheb, xk, xkm1, xkp1, yout, nullptr,
+                                   mu_d + k, mu_d, k == 1 ? t1 : t2, cc, k == 1, nullptr,
+                                   c.stream));
+      c.astats.filter_bytes += filter_step_bytes(c, 1, (int)vs, (int)vs, k == 1 ? 3 : 5);
+    }
+    if (f32) EVSL_HIP(evsl_dev::f32_to_f64(n, (const float*)yout, y, c.stream));
+    c.astats.n_filter_steps += m;
+    c.astats.n_filter_cols += m;
+    return 0;
+  }
+  // user operator: y = mu0 v; per degree vkp1 = A vk then a fused update
+  EVSL_HIP(w.bufs.alloc((size_t)4 * n * sizeof(double)));
+  double* b = w.bufs.as<double>();
+  double* buf[3] = {b, b + n, b + 2 * (size_t)n};
+  double* tmp = b + 3 * (size_t)n;
+  EVSL_HIP(evsl_dev::scal_copy(n, pol->mu[0], v, y, c.stream));
+  for (int k = 1; k <= m; ++k) {
+    double* xkp1 = buf[(k - 1) % 3];
+    const double* xk = (k == 1) ? v : buf[(k - 2) % 3];
+    const double* xkm1 = (k == 1) ? nullptr : (k == 2 ? v : buf[(k - 3) % 3]);
+    if (int e = apply_A(xk, tmp)) return e;
+    // xkp1 = t*(tmp - cc*xk) - xkm1 ; y += mu_k xkp1
+    const double t = (k == 1 ? t1 : t2);
+    EVSL_HIP(evsl_dev::scal_copy(n, t, tmp, xkp1, c.stream));
+    EVSL_HIP(evsl_dev::axpy_dot(n, -t * cc, xk, xkp1, nullptr, nullptr, c.stream));
+    if (xkm1) EVSL_HIP(evsl_dev::axpy_dot(n, -1.0, xkm1, xkp1, nullptr, nullptr, c.stream));
+    EVSL_HIP(evsl_dev::axpy_dot(n, pol->mu[k], xkp1, y, nullptr, nullptr, c.stream));
+  }
+  c.astats.n_filter_steps += m;
+  c.astats.n_filter_cols += m;
+  return 0;
+}
+
+namespace {
+
+// Workspace of one filtered Lanczos run
+struct LanWork {
+  DevBuf V, gt_part, h, vrand;
+  ChebWork cw;
+  int nch = 0;
+};
+
+// CGS with the DGKS test, two GEMV passes at most (SRC/misc_la.c:221-261).
+// On entry *old2 = ||v||^2 (already known); returns ||v|| in *nrm.
+int cgs_dgks(int n, int k, const double* Q, double* v, double old2, double* nrm, LanWork& L) {
+  Context& c = ctx();
+  const double eta = 1.0 / std::sqrt(2.0);
+  double old_nrm = std::sqrt(old2), new_nrm = 0.0;
+  const int nb = (n + evsl_dev::kWG - 1) / evsl_dev::kWG;
+  for (int it = 0; it < kNgsMax; ++it) {
+    EVSL_HIP(evsl_dev::gemv_t(n, k, Q, n, v, L.gt_part.as<double>(), L.h.as<double>(), c.stream));
+    EVSL_HIP(evsl_dev::gemv_n_sub_nrm(n, k, Q, n, L.h.as<double>(), v, red_part(), c.stream));
+    EVSL_HIP(evsl_dev::sum_scalar(red_part(), nb, red_scalar(2), c.stream));
+    double s2;
+    if (int e = read_scalars(red_scalar(2), 1, &s2)) return e;
+    new_nrm = std::sqrt(s2);
+    c.astats.n_reorth_pass++;
+    c.astats.reorth_bytes += 16.0 * n * k + 24.0 * n;
+    if (new_nrm > eta * old_nrm) break;
+    old_nrm = new_nrm;
+  }
+  *nrm = new_nrm;
+  return 0;
+}
+
+int nrm2_dev(int n, const double* x, double* out) {
+  Context& c = ctx();
+  EVSL_HIP(evsl_dev::dot_partial(n, x, x, red_part(), c.stream));
+  EVSL_HIP(evsl_dev::sum_scalar(red_part(), evsl_dev::kRedBlocks, red_scalar(0), c.stream));
+  double s;
+  if (int e = read_scalars(red_scalar(0), 1, &s)) return e;
+  *out = std::sqrt(s);
+  return 0;
+}
+
+}  // namespace
+
+// Ritz extraction shared by the filtered Lanczos solvers: given the
+// tridiagonal T_kdim (dT, eT) and the basis V, compute Ritz pairs of A for
+// the eigenvalues of T at or above bar, keep those in [a, b] with residual
+// below tol (SRC/cheblanNr.c:316-399).
+static int ritz_extract(int n, int kdim, const double* dT, const double* eT, const double* V,
+                        double bar, double aa, double bb, double tol, int* nevOut,
+                        double** lamo, double** Wo, double** reso, int maxit_alloc) {
+  Context& c = ctx();
+  std::vector<double> eval(kdim), e(kdim + 1, 0.0);
+  std::memcpy(eval.data(), dT, kdim * sizeof(double));
+  if (kdim > 1) std::memcpy(e.data(), eT, (kdim - 1) * sizeof(double));
+  RotationLog rec;
+  if (int err = tridiag_ql(kdim, eval.data(), e.data(), &rec)) return err;
+  std::vector<int> sel;
+  for (int i = 0; i < kdim; ++i)
+    if (!(eval[i] < bar)) sel.push_back(i);
+  const int m = (int)sel.size();
+  double* Lam = (double*)std::malloc((size_t)std::max(maxit_alloc, 1) * sizeof(double));
+  double* res = (double*)std::malloc((size_t)std::max(maxit_alloc, 1) * sizeof(double));
+  *lamo = Lam;
+  *reso = res;
+  *Wo = nullptr;
+  *nevOut = 0;
+  if (m == 0) return 0;
+  DevBuf Zsel, U, W, part, s2, tv, r2;
+  EVSL_HIP(Zsel.alloc((size_t)kdim * m * sizeof(double)));
+  if (int err = tridiag_rotations_to_device(kdim, rec, sel, Zsel.as<double>())) return err;
+  EVSL_HIP(U.alloc((size_t)n * m * sizeof(double)));
+  EVSL_HIP(W.alloc((size_t)n * m * sizeof(double)));
+  const int nch = (n + 8191) / 8192;
+  EVSL_HIP(part.alloc((size_t)nch * m * sizeof(double)));
+  EVSL_HIP(s2.alloc(m * sizeof(double)));
+  EVSL_HIP(tv.alloc(m * sizeof(double)));
+  EVSL_HIP(r2.alloc(m * sizeof(double)));
+  double* Ud = U.as<double>();
+  double* Wd = W.as<double>();
+  // U = V(:, 0:kdim) * Zsel
+  EVSL_HIP(evsl_dev::gemm_nn(n, m, kdim, V, n, Zsel.as<double>(), kdim, Ud, n, c.stream));
+  // normalise the columns
+  EVSL_HIP(evsl_dev::col_dot(n, m, Ud, n, Ud, n, part.as<double>(), s2.as<double>(), c.stream));
+  EVSL_HIP(evsl_dev::col_scale_rsqrt(n, m, Ud, n, s2.as<double>(), c.stream));
+  // W = A U, Rayleigh quotients, residual norms
+  for (int j = 0; j < m; ++j)
+    if (int err = apply_A(Ud + (size_t)j * n, Wd + (size_t)j * n)) return err;
+  EVSL_HIP(evsl_dev::col_dot(n, m, Wd, n, Ud, n, part.as<double>(), tv.as<double>(), c.stream));
+  EVSL_HIP(evsl_dev::col_res2(n, m, Wd, n, Ud, n, tv.as<double>(), part.as<double>(),
+                              r2.as<double>(), c.stream));
+  std::vector<double> hs2(m), ht(m), hr2(m);
+  EVSL_HIP(hipMemcpy(hs2.data(), s2.p, m * sizeof(double), hipMemcpyDeviceToHost));
+  EVSL_HIP(hipMemcpy(ht.data(), tv.p, m * sizeof(double), hipMemcpyDeviceToHost));
+  EVSL_HIP(hipMemcpy(hr2.data(), r2.p, m * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<int> keep;
+  int nev = 0;
+  for (int j = 0; j < m; ++j) {
+    if (hs2[j] == 0.0) return 2;  // zero Ritz vector (SRC/cheblanNr.c:362-364)
+    const double t = ht[j];
+    if (t < aa - kEpsM || t > bb + kEpsM) continue;
+    const double r = std::sqrt(std::max(hr2[j], 0.0));
+    if (r < tol) {
+      Lam[nev] = t;
+      res[nev] = r;
+      keep.push_back(j);
+      ++nev;
+    }
+  }
+  *nevOut = nev;
+  if (nev > 0) {
+    double* Y = (double*)_evsl_Malloc_device((size_t)n * nev * sizeof(double));
+    if (!Y) return EVSL_ERR_ALLOC;
+    DevBuf idx;
+    EVSL_HIP(idx.alloc(nev * sizeof(int)));
+    EVSL_HIP(hipMemcpy(idx.p, keep.data(), nev * sizeof(int), hipMemcpyHostToDevice));
+    EVSL_HIP(evsl_dev::gather_cols(n, nev, Ud, n, idx.as<int>(), Y, n, c.stream));
+    EVSL_HIP(hipStreamSynchronize(c.stream));
+    *Wo = Y;
+  }
+  return 0;
+}
+
+}  // namespace evsl
+
+using namespace evsl;
+
+extern "C" {
+
+// SRC/cheblanNr.c:57-434
+int ChebLanNr(double* intv, int maxit, double tol, double* vinit, polparams* pol, int* nevOut,
+              double** lamo, double** Wo, double** reso, FILE* fstats) {
+  if (int e = ensure_started()) return e;
+  Context& c = ctx();
+  if (c.op == 0) return EVSL_ERR_NOMAT;
+  const double t_all = evsl_timer();
+  const int n = c.n;
+  maxit = std::min(n, maxit);
+  const double bar = pol->bar, gamB = pol->gam;
+  if (check_intv(intv) < 0) {
+    *nevOut = 0;
+    *lamo = nullptr;
+    *Wo = nullptr;
+    *reso = nullptr;
+    return 0;
+  }
+  const double aa = intv[0], bb = intv[1];
+  if (fstats)
+    std::fprintf(fstats, " ** Cheb Poly of deg = %d, gam = %.15e, bar: %.15e\n", pol->deg,
+                 gamB, bar);
+  if (gamB > 1.0 || gamB < -1.0) {
+    std::fprintf(stdout, "gamB error %.15e\n", gamB);
+    return -1;
+  }
+  const int prec = c.filter_prec;
+  LanWork L;
+  const size_t nl = n;
+  EVSL_HIP(L.V.alloc(nl * (maxit + 1) * sizeof(double)));
+  L.nch = evsl_dev::gemv_t_chunks(n);
+  EVSL_HIP(L.gt_part.alloc((size_t)L.nch * (maxit + 1) * sizeof(double)));
+  EVSL_HIP(L.h.alloc((size_t)(maxit + 1) * sizeof(double)));
+  EVSL_HIP(L.vrand.alloc(nl * sizeof(double)));
+  double* V = L.V.as<double>();
+  std::vector<double> dT(maxit), eT(maxit), EvalT(maxit);
+  DevBuf vtmp;
+  const double* v0;
+  if (int e = as_device_vector(vinit, n, vtmp, &v0)) return e;
+  EventClock clk;
+  // filter the start vector and normalise it (FILTER_VINIT)
+  clk.mark(0);
+  if (int e = cheb_av(pol, v0, V, L.cw, prec)) return e;
+  clk.mark(1);
+  double nv;
+  if (int e = nrm2_dev(n, V, &nv)) return e;
+  c.astats.t_filter += clk.secs(0, 1);
+  EVSL_HIP(evsl_dev::scal(n, 1.0 / nv, V, c.stream));
+  double wn = 0.0, beta = 0.0, tr0 = 0.0;
+  int nwn = 0, kdim = 0, k;
+  const int Ntest = 30, cycle = 20;
+  for (k = 0; k < maxit; ++k) {
+    double* v = V + (size_t)k * nl;
+    double* vnew = v + nl;
+    const double* zold = k > 0 ? v - nl : nullptr;
+    clk.mark(0);
+    if (int e = cheb_av(pol, v, vnew, L.cw, prec)) return e;
+    clk.mark(1);
+    // vnew -= beta*zold; alpha = v.vnew; vnew -= alpha*v; ||vnew||^2
+    if (zold)
+      EVSL_HIP(evsl_dev::axpy_dot(n, -beta, zold, vnew, v, red_part(), c.stream));
+    else
+      EVSL_HIP(evsl_dev::dot_partial(n, vnew, v, red_part(), c.stream));
+    EVSL_HIP(evsl_dev::sum_scalar(red_part(), evsl_dev::kRedBlocks, red_scalar(0), c.stream));
+    EVSL_HIP(evsl_dev::axpy_dev_nrm(n, -1.0, red_scalar(0), v, vnew, red_part(), c.stream));
+    EVSL_HIP(evsl_dev::sum_scalar(red_part(), evsl_dev::kRedBlocks, red_scalar(1), c.stream));
+    double sc[2];
+    if (int e = read_scalars(red_scalar(0), 2, sc)) return e;
+    const double alpha = sc[0];
+    dT[k] = alpha;
+    wn += std::fabs(alpha);
+    if (int e = cgs_dgks(n, k + 1, V, vnew, sc[1], &beta, L)) return e;
+    clk.mark(2);
+    wn += 2.0 * beta;
+    nwn += 3;
+    if (beta * nwn < kOrthTol * wn) {  // lucky breakdown (SRC/cheblanNr.c:225-256)
+      if (fstats) std::fprintf(fstats, "it %4d: Lucky breakdown, beta = %.15e\n", k, beta);
+      rand_double_device(n, L.vrand.as<double>());
+      if (int e = cheb_av(pol, L.vrand.as<double>(), vnew, L.cw, prec)) return e;
+      double nn;
+      if (int e = nrm2_dev(n, vnew, &nn)) return e;
+      if (int e = cgs_dgks(n, k + 1, V, vnew, nn * nn, &beta, L)) return e;
+      EVSL_HIP(evsl_dev::scal(n, 1.0 / beta, vnew, c.stream));
+      beta = 0.0;
+    } else {
+      EVSL_HIP(evsl_dev::scal(n, 1.0 / beta, vnew, c.stream));
+    }
+    eT[k] = beta;
+    c.astats.t_filter += clk.secs(0, 1);
+    c.astats.t_reorth += clk.secs(1, 2);
+    c.astats.n_lanczos++;
+    if ((k < Ntest || (k - Ntest) % cycle != 0) && k != maxit - 1) continue;
+    // restricted-trace convergence test (SRC/cheblanNr.c:269-309)
+    kdim = k + 1;
+    const double te = evsl_timer();
+    std::vector<double> e(kdim + 1, 0.0);
+    std::memcpy(EvalT.data(), dT.data(), kdim * sizeof(double));
+    std::memcpy(e.data(), eT.data(), (kdim - 1) * sizeof(double));
+    if (tridiag_ql(kdim, EvalT.data(), e.data(), nullptr)) return EVSL_ERR_EIG;
+    c.astats.t_eig += evsl_timer() - te;
+    double tr1 = 0.0;
+    int nconv = 0;
+    for (int i = 0; i < kdim; ++i)
+      if (EvalT[i] + kEpsM >= bar) {
+        tr1 += EvalT[i];
+        ++nconv;
+      }
+    if (fstats) std::fprintf(fstats, "k %4d: nconv %4d  tr1 %21.15e\n", k, nconv, tr1);
+    if (std::fabs(tr1 - tr0) < 2e-12 || std::fabs(tr1) + std::fabs(tr0) < 1e-10) break;
+    tr0 = tr1;
+  }
+  if (k >= maxit && fstats)
+    std::fprintf(fstats, "The max number of iterations [%d] has been reached. The eigenvalues "
+                         "computed may not have converged\n", maxit);
+  const double tr = evsl_timer();
+  int err = ritz_extract(n, kdim, dT.data(), eT.data(), V, bar, aa, bb, tol, nevOut, lamo, Wo,
+                         reso, maxit);
+  const double tnow = evsl_timer();
+  c.astats.t_ritz += tnow - tr;
+  evslstat.t_ritz = tnow - tr;
+  evslstat.t_iter = tnow - t_all;
+  return err;
+}
+
+// Lanczos with full (modified Gram-Schmidt) re-orthogonalization giving
+// spectrum bounds (SRC/lanbounds.c:21-144).  v may be a host or device vector.
+int LanBounds(int msteps, double* vin, double* lmin, double* lmax) {
+  if (int e = ensure_started()) return e;
+  Context& c = ctx();
+  if (c.op == 0) return EVSL_ERR_NOMAT;
+  const int n = c.n;
+  const size_t nl = n;
+  msteps = std::min(n, msteps);
+  std::vector<double> alp(msteps), bet(msteps);
+  DevBuf Vb, vtmp;
+  EVSL_HIP(Vb.alloc((msteps + 1) * nl * sizeof(double)));
+  double* V = Vb.as<double>();
+  const double* v;
+  if (int e = as_device_vector(vin, n, vtmp, &v)) return e;
+  double nv;
+  if (int e = nrm2_dev(n, v, &nv)) return e;
+  EVSL_HIP(evsl_dev::scal_copy(n, 1.0 / nv, v, V, c.stream));
+  double wn = 0.0;
+  double* part = red_part();
+  for (int j = 0; j < msteps; ++j) {
+    double* vj = V + j * nl;
+    double* vn = vj + nl;
+    if (int e = apply_A(vj, vn)) return e;
+    if (j)
+      EVSL_HIP(evsl_dev::axpy_dot(n, -bet[j - 1], vj - nl, vn, vj, part, c.stream));
+    else
+      EVSL_HIP(evsl_dev::dot_partial(n, vn, vj, part, c.stream));
+    EVSL_HIP(evsl_dev::sum_scalar(part, evsl_dev::kRedBlocks, red_scalar(0), c.stream));
+    // vn -= alp vj, then modified Gram-Schmidt against V(:,0..j): each step
+    // subtracts the projection computed by the previous launch
+    EVSL_HIP(evsl_dev::axpy_dev_dot(n, -1.0, red_scalar(0), vj, vn, V, part, c.stream));
+    for (int i = 0; i <= j; ++i) {
+      EVSL_HIP(evsl_dev::sum_scalar(part, evsl_dev::kRedBlocks, red_scalar(1), c.stream));
+      const double* next = (i < j) ? V + (i + 1) * nl : vn;
+      EVSL_HIP(evsl_dev::axpy_dev_dot(n, -1.0, red_scalar(1), V + i * nl, vn, next, part,
+                                      c.stream));
+    }
+    EVSL_HIP(evsl_dev::sum_scalar(part, evsl_dev::kRedBlocks, red_scalar(2), c.stream));
+    double a_b[3];
+    if (int e = read_scalars(red_scalar(0), 3, a_b)) return e;
+    alp[j] = a_b[0];
+    bet[j] = a_b[2];
+    wn += alp[j] * alp[j];
+    if (bet[j] * (j + 1) < kOrthTol * wn) {
+      std::fprintf(stdout, "lanbounds: lucky break, j=%d, beta=%e, break\n", j, bet[j]);
+      msteps = j + 1;
+      break;
+    }
+    wn += 2.0 * bet[j];
+    bet[j] = std::sqrt(bet[j]);
+    EVSL_HIP(evsl_dev::scal(n, 1.0 / bet[j], vn, c.stream));
+  }
+  const double bottomBeta = bet[msteps - 1];
+  std::vector<double> S((size_t)msteps * msteps), ritz(msteps);
+  if (tridiag_eig_host(msteps, alp.data(), bet.data(), ritz.data(), S.data()))
+    return EVSL_ERR_EIG;
+  *lmin = ritz[0] - std::fabs(bottomBeta * S[msteps - 1]);
+  *lmax = ritz[msteps - 1] + std::fabs(bottomBeta * S[(size_t)msteps * msteps - 1]);
+  return 0;
+}
+
+// Lanczos DOS: nvec independent Lanczos runs with full re-orthogonalization,
+// Gauss quadrature nodes/weights smoothed by Gaussians, Simpson integral
+// (SRC/landos.c:37-196).  Random vectors are Box-Muller draws from libc
+// rand() as in the reference.
+int LanDos(const int nvec, int msteps, int npts, double* xdos, double* ydos, double* neig,
+           const double* const intv) {
+  if (int e = ensure_started()) return e;
+  Context& c = ctx();
+  if (c.op == 0) return EVSL_ERR_NOMAT;
+  const int n = c.n;
+  const size_t nl = n;
+  const size_t ml0 = msteps;
+  std::vector<double> y(npts, 0.0), S(ml0 * ml0), gamma2(msteps), ritz(msteps);
+  std::vector<int> ind(npts);
+  const double lm = intv[2], lM = intv[3];
+  const double tolBdwn = 1.e-13 * (std::fabs(lM) + std::fabs(lm));
+  const double aa = std::max(intv[0], intv[2]), bb = std::min(intv[1], intv[3]);
+  const double kappa = 1.25;
+  const int M = std::min(msteps, 30);
+  const double H = (lM - lm) / (M - 1);
+  const double sigma = H / std::sqrt(8 * std::log(kappa));
+  const double sigma2 = 2 * sigma * sigma;
+  const double width = sigma * std::sqrt(-2.0 * std::log(1e-08));
+  linspace(aa, bb, npts, xdos);
+  std::vector<double> alp(msteps), bet(msteps), hv(nl);
+  DevBuf Vb, vb;
+  EVSL_HIP(Vb.alloc((msteps + 1) * nl * sizeof(double)));
+  EVSL_HIP(vb.alloc(nl * sizeof(double)));
+  double* V = Vb.as<double>();
+  double* part = red_part();
+  for (int m = 0; m < nvec; ++m) {
+    randn_double(n, hv.data());
+    EVSL_HIP(hipMemcpy(vb.p, hv.data(), nl * sizeof(double), hipMemcpyHostToDevice));
+    double nv;
+    if (int e = nrm2_dev(n, vb.as<double>(), &nv)) return e;
+    EVSL_HIP(evsl_dev::scal_copy(n, 1.0 / nv, vb.as<double>(), V, c.stream));
+    double wn = 0.0;
+    for (int j = 0; j < msteps; ++j) {
+      double* vj = V + j * nl;
+      double* vn = vj + nl;
+      if (int e = apply_A(vj, vn)) return e;
+      if (j)
+        EVSL_HIP(evsl_dev::axpy_dot(n, -bet[j - 1], vj - nl, vn, vj, part, c.stream));
+      else
+        EVSL_HIP(evsl_dev::dot_partial(n, vn, vj, part, c.stream));
+      EVSL_HIP(evsl_dev::sum_scalar(part, evsl_dev::kRedBlocks, red_scalar(0), c.stream));
+      EVSL_HIP(evsl_dev::axpy_dev_dot(n, -1.0, red_scalar(0), vj, vn, V, part, c.stream));
+      for (int i = 0; i <= j; ++i) {
+        EVSL_HIP(evsl_dev::sum_scalar(part, evsl_dev::kRedBlocks, red_scalar(1), c.stream));
+        const double* next = (i < j) ? V + (i + 1) * nl : vn;
+        EVSL_HIP(evsl_dev::axpy_dev_dot(n, -1.0, red_scalar(1), V + i * nl, vn, next, part,
+                                        c.stream));
+      }
+      EVSL_HIP(evsl_dev::sum_scalar(part, evsl_dev::kRedBlocks, red_scalar(2), c.stream));
+      double a_b[3];
+      if (int e = read_scalars(red_scalar(0), 3, a_b)) return e;
+      alp[j] = a_b[0];
+      bet[j] = a_b[2];
+      wn += alp[j] * alp[j];
+      if (bet[j] * (j + 1) < kOrthTol * wn) {
+        std::fprintf(stdout, "lanbounds: lucky break, j=%d, beta=%e, break\n", j, bet[j]);
+        msteps = j + 1;
+        break;
+      }
+      if (bet[j] > tolBdwn) {
+        wn += 2.0 * bet[j];
+        bet[j] = std::sqrt(bet[j]);
+        EVSL_HIP(evsl_dev::scal(n, 1.0 / bet[j], vn, c.stream));
+      } else {
+        // the reference draws the replacement into v, which is then unused
+        // (SRC/landos.c:130-144); the stream of rand() calls is kept the same
+        randn_double(n, hv.data());
+        EVSL_HIP(evsl_dev::dot_partial(n, vn, V, part, c.stream));
+        for (int i = 0; i <= j; ++i) {
+          EVSL_HIP(evsl_dev::sum_scalar(part, evsl_dev::kRedBlocks, red_scalar(1), c.stream));
+          const double* next = (i < j) ? V + (i + 1) * nl : vn;
+          EVSL_HIP(evsl_dev::axpy_dev_dot(n, -1.0, red_scalar(1), V + i * nl, vn, next, part,
+                                          c.stream));
+        }
+        EVSL_HIP(evsl_dev::sum_scalar(part, evsl_dev::kRedBlocks, red_scalar(2), c.stream));
+        double b2;
+        if (int e = read_scalars(red_scalar(2), 1, &b2)) return e;
+        wn += 2.0 * b2;
+        const double bj = std::sqrt(b2);
+        EVSL_HIP(evsl_dev::scal(n, 1.0 / bj, vn, c.stream));
+        bet[j] = 0;
+      }
+    }
+    if (tridiag_eig_host(msteps, alp.data(), bet.data(), ritz.data(), S.data()))
+      return EVSL_ERR_EIG;
+    for (int i = 0; i < msteps; ++i) gamma2[i] = S[i * ml0] * S[i * ml0];
+    for (int i = 0; i < msteps; ++i) {
+      const double t = ritz[i];
+      int np = 0;
+      for (int j = 0; j < npts; ++j)
+        if (std::fabs(xdos[j] - t) < width) ind[np++] = j;
+      for (int j = 0; j < np; ++j)
+        y[ind[j]] += gamma2[i] * std::exp(-((xdos[ind[j]] - t) * (xdos[ind[j]] - t)) / sigma2);
+    }
+  }
+  const double scaling = 1.0 / (nvec * std::sqrt(sigma2 * EVSL_PI));
+  for (int j = 0; j < npts; ++j) y[j] *= scaling;
+  std::memcpy(ydos, y.data(), npts * sizeof(double));
+  // in-place Simpson integration (SRC/simpson.c:25-50)
+  {
+    double tc = 0.0, ti, tm, ysav = y[0];
+    int i = 1;
+    y[0] = 0;
+    while (i < npts - 1) {
+      ti = (xdos[i] - xdos[i - 1]) * (ysav + y[i]) / 2.0;
+      tm = tc + ti;
+      ti = (xdos[i + 1] - xdos[i - 1]) * (ysav + 4 * y[i] + y[i + 1]) / 6.0;
+      tc += ti;
+      y[i] = tm;
+      ysav = y[i + 1];
+      y[i + 1] = tc;
+      i += 2;
+    }
+    if (i == npts - 1) y[i] = tc + (xdos[i] - xdos[i - 1]) * (y[i] + ysav) / 2.0;
+  }
+  *neig = y[npts - 1] * n;
+  return 0;
+}
+
+// Thick-restart filtered Lanczos: not yet on the device path.
+int ChebLanTr(int, int, double*, int, double, double*, polparams*, int* nev2, double** vals,
+              double** W, double** resW, FILE*) {
+  *nev2 = 0;
+  *vals = nullptr;
+  *W = nullptr;
+  *resW = nullptr;
+  set_error_msg("ChebLanTr: not available in this build");
+  return EVSL_ERR_UNSUPP;
+}
+
+}  // extern "C"

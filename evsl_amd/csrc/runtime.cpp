@@ -1,0 +1,425 @@
+// runtime.cpp -- context, memory helpers, operator registration, RNG,
+// timing and statistics of the drop-in C interface.
+//
+// Mirrors SRC/evsl.c (EVSLStart/SetAMatrix/SetAMatvec), SRC/evsl_memory.c,
+// SRC/timing.c, SRC/stats.c and SRC/vect.c:14-61 of EVSL 1.1.1.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <string>
+
+#include "internal.h"
+
+evslStat evslstat;
+
+namespace evsl {
+
+static std::string g_err;
+
+void set_error(const char* where, hipError_t e) {
+  if (g_err.empty()) g_err = std::string(where) + ": " + hipGetErrorString(e);
+}
+void set_error_msg(const std::string& msg) {
+  if (g_err.empty()) g_err = msg;
+}
+const char* last_error() { return g_err.c_str(); }
+
+hipError_t DevBuf::alloc(size_t nbytes) {
+  if (nbytes <= bytes && p) return hipSuccess;
+  release();
+  if (nbytes == 0) return hipSuccess;
+  hipError_t e = hipMalloc(&p, nbytes);
+  if (e != hipSuccess) {
+    p = nullptr;
+    return e;
+  }
+  bytes = nbytes;
+  return hipSuccess;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+evsl_dev::Csr DevMatrix::view() const {
+  evsl_dev::Csr c;
+  c.n = n;
+  c.nnz = nnz;
+  c.ia = ia;
+  c.ja = ja;
+  c.a64 = a64;
+  c.a32 = a32;
+  c.rb = rb;
+  c.nblk = nblk;
+  return c;
+}
+
+void DevMatrix::release() {
+  if (owns_csr) {
+    if (ia) (void)hipFree(ia);
+    if (ja) (void)hipFree(ja);
+    if (a64) (void)hipFree(a64);
+  }
+  if (rb) (void)hipFree(rb);
+  if (a32) (void)hipFree(a32);
+  *this = DevMatrix();
+}
+
+std::vector<int> make_row_blocks(const int* ia, int n) {
+  std::vector<int> rb;
+  rb.reserve(n / 64 + 2);
+  int r = 0;
+  while (r < n) {
+    rb.push_back(r);
+    const int len0 = ia[r + 1] - ia[r];
+    if (len0 > evsl_dev::kNnzCap) {  // a long row gets a block of its own
+      r += 1;
+      continue;
+    }
+    int rows = 0, nz = 0;
+    while (r < n && rows < evsl_dev::kRowsCap) {
+      const int len = ia[r + 1] - ia[r];
+      if (len > evsl_dev::kNnzCap || nz + len > evsl_dev::kNnzCap) break;
+      nz += len;
+      ++rows;
+      ++r;
+    }
+  }
+  rb.push_back(n);
+  return rb;
+}
+
+static Context g_ctx;
+Context& ctx() { return g_ctx; }
+
+int ensure_started() {
+  Context& c = g_ctx;
+  if (c.started) return 0;
+  return EVSLStart();
+}
+
+double* red_part() { return g_ctx.red_part.as<double>(); }
+double* red_scalar(int i) { return g_ctx.red_scalars.as<double>() + i; }
+
+int read_scalars(const double* d, int cnt, double* out) {
+  Context& c = g_ctx;
+  EVSL_HIP(hipMemcpyAsync(c.pinned, d, cnt * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  EVSL_HIP(hipStreamSynchronize(c.stream));
+  std::memcpy(out, c.pinned, cnt * sizeof(double));
+  return 0;
+}
+
+int ensure_a32() {
+  Context& c = g_ctx;
+  if (c.op != 1) return EVSL_ERR_NOMAT;
+  if (c.A.a32) return 0;
+  EVSL_HIP(hipMalloc((void**)&c.A.a32, (size_t)std::max(c.A.nnz, 1) * sizeof(float)));
+  EVSL_HIP(evsl_dev::f64_to_f32(c.A.nnz, c.A.a64, c.A.a32, c.stream));
+  return 0;
+}
+
+int apply_A(const double* x, double* y) {
+  Context& c = g_ctx;
+  if (c.op == 1) {
+    EVSL_HIP(evsl_dev::cheb_spmm(c.A.view(), 1, false, evsl_dev::kPlain, x, nullptr, y,
+                                 nullptr, nullptr, nullptr, nullptr, 1.0, 0.0, false, nullptr,
+                                 c.stream));
+    return 0;
+  }
+  if (c.op == 2) {
+    EVSL_HIP(hipStreamSynchronize(c.stream));
+    c.func(const_cast<double*>(x), y, c.func_data);
+    EVSL_HIP(hipDeviceSynchronize());
+    return 0;
+  }
+  set_error_msg("no operator: call SetAMatrix or SetAMatvec first");
+  return EVSL_ERR_NOMAT;
+}
+
+// register a device CSR (arrays already on the device); builds row blocks
+static int register_device_csr(int n, int nnz, int* ia_d, int* ja_d, double* a_d,
+                               const int* ia_h, bool owns) {
+  Context& c = g_ctx;
+  c.A.release();
+  c.A.n = n;
+  c.A.nnz = nnz;
+  c.A.ia = ia_d;
+  c.A.ja = ja_d;
+  c.A.a64 = a_d;
+  c.A.owns_csr = owns;
+  std::vector<int> ia_tmp;
+  if (!ia_h) {
+    ia_tmp.resize(n + 1);
+    EVSL_HIP(hipMemcpy(ia_tmp.data(), ia_d, (n + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    ia_h = ia_tmp.data();
+  }
+  std::vector<int> rb = make_row_blocks(ia_h, n);
+  c.A.nblk = (int)rb.size() - 1;
+  EVSL_HIP(hipMalloc((void**)&c.A.rb, rb.size() * sizeof(int)));
+  EVSL_HIP(hipMemcpy(c.A.rb, rb.data(), rb.size() * sizeof(int), hipMemcpyHostToDevice));
+  c.n = n;
+  c.op = 1;
+  // reduction scratch sized for the largest per-block partial buffers
+  size_t need = (size_t)std::max(evsl_dev::kRedBlocks * 64, (c.A.nblk + 1) * 32);
+  EVSL_HIP(c.red_part.alloc(need * sizeof(double)));
+  return 0;
+}
+
+}  // namespace evsl
+
+using namespace evsl;
+
+extern "C" {
+
+// ---- memory (SRC/evsl_memory.c:9-149) -----------------------------------
+void* _evsl_Malloc(size_t nbytes) { return std::malloc(nbytes); }
+void* _evsl_Calloc(size_t count, size_t nbytes) { return std::calloc(count, nbytes); }
+void* _evsl_Realloc(void* ptr, size_t nbytes) { return std::realloc(ptr, nbytes); }
+void _evsl_Free(void* ptr) { std::free(ptr); }
+
+void* _evsl_Malloc_device(size_t nbytes) {
+  void* p = nullptr;
+  if (nbytes == 0) return nullptr;
+  hipError_t e = hipMalloc(&p, nbytes);
+  if (e != hipSuccess) {
+    set_error("hipMalloc", e);
+    return nullptr;
+  }
+  return p;
+}
+
+void* _evsl_Calloc_device(size_t count, size_t nbytes) {
+  void* p = _evsl_Malloc_device(count * nbytes);
+  if (p && hipMemset(p, 0, count * nbytes) != hipSuccess) {
+    (void)hipFree(p);
+    return nullptr;
+  }
+  return p;
+}
+
+void* _evsl_Realloc_device(void* old_ptr, size_t old_nbytes, size_t new_nbytes) {
+  void* p = _evsl_Malloc_device(new_nbytes);
+  if (p && old_ptr) {
+    (void)hipMemcpy(p, old_ptr, std::min(old_nbytes, new_nbytes), hipMemcpyDeviceToDevice);
+    (void)hipFree(old_ptr);
+  }
+  return p;
+}
+
+void _evsl_Free_device(void* ptr) {
+  if (ptr) (void)hipFree(ptr);
+}
+
+void evsl_memcpy_device_to_host(void* dst, void* src, size_t nbytes) {
+  hipError_t e = hipMemcpy(dst, src, nbytes, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) set_error("hipMemcpy D2H", e);
+}
+void evsl_memcpy_device_to_device(void* dst, void* src, size_t nbytes) {
+  hipError_t e = hipMemcpy(dst, src, nbytes, hipMemcpyDeviceToDevice);
+  if (e != hipSuccess) set_error("hipMemcpy D2D", e);
+}
+void evsl_memcpy_host_to_device(void* dst, void* src, size_t nbytes) {
+  hipError_t e = hipMemcpy(dst, src, nbytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) set_error("hipMemcpy H2D", e);
+}
+void evsl_memset_device(void* ptr, int value, size_t nbytes) {
+  hipError_t e = hipMemset(ptr, value, nbytes);
+  if (e != hipSuccess) set_error("hipMemset", e);
+}
+
+// ---- lifecycle (SRC/evsl.c:24-93) -----------------------------------------
+int EVSLStart(void) {
+  Context& c = ctx();
+  if (c.started) return 0;
+  int dev = 0;
+  EVSL_HIP(hipGetDevice(&dev));
+  EVSL_HIP(hipFree(nullptr));  // create the primary context
+  c.stream = nullptr;          // legacy null stream
+  EVSL_HIP(c.red_part.alloc((size_t)evsl_dev::kRedBlocks * 64 * sizeof(double)));
+  EVSL_HIP(c.red_scalars.alloc(64 * sizeof(double)));
+  EVSL_HIP(hipHostMalloc((void**)&c.pinned, 4096 * sizeof(double), hipHostMallocDefault));
+  c.started = true;
+  StatsReset();
+  std::memset(&c.astats, 0, sizeof(c.astats));
+  return 0;
+}
+
+int EVSLFinish(void) {
+  Context& c = ctx();
+  if (!c.started) return 0;
+  (void)hipDeviceSynchronize();
+  c.A.release();
+  c.red_part.release();
+  c.red_scalars.release();
+  if (c.pinned) (void)hipHostFree(c.pinned);
+  c.pinned = nullptr;
+  c.op = 0;
+  c.n = 0;
+  c.func = nullptr;
+  c.func_data = nullptr;
+  c.started = false;
+  return 0;
+}
+
+// Host CSR: uploaded to the device (FP64 values; FP32 copy made on demand).
+// The reference keeps the pointer (SRC/evsl.c:95-101); here the device copy
+// is taken at this call, so later host edits of A are not seen.
+int SetAMatrix(csrMat* A) {
+  if (int e = ensure_started()) return e;
+  if (!A || A->nrows != A->ncols || A->nrows <= 0) return EVSL_ERR_ARG;
+  const int n = A->nrows, nnz = A->ia[n];
+  int *ia = nullptr, *ja = nullptr;
+  double* a = nullptr;
+  EVSL_HIP(hipMalloc((void**)&ia, (n + 1) * sizeof(int)));
+  EVSL_HIP(hipMalloc((void**)&ja, (size_t)std::max(nnz, 1) * sizeof(int)));
+  EVSL_HIP(hipMalloc((void**)&a, (size_t)std::max(nnz, 1) * sizeof(double)));
+  EVSL_HIP(hipMemcpy(ia, A->ia, (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+  EVSL_HIP(hipMemcpy(ja, A->ja, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice));
+  EVSL_HIP(hipMemcpy(a, A->a, (size_t)nnz * sizeof(double), hipMemcpyHostToDevice));
+  return register_device_csr(n, nnz, ia, ja, a, A->ia, true);
+}
+
+// A->ia/ja/a are device pointers (as produced by evsl_create_csr_gpu)
+int SetAMatrix_device_csr(csrMat* A) {
+  if (int e = ensure_started()) return e;
+  if (!A || A->nrows != A->ncols || A->nrows <= 0) return EVSL_ERR_ARG;
+  int nnz = 0;
+  EVSL_HIP(hipMemcpy(&nnz, A->ia + A->nrows, sizeof(int), hipMemcpyDeviceToHost));
+  return register_device_csr(A->nrows, nnz, A->ia, A->ja, A->a, nullptr, false);
+}
+
+void evsl_create_csr_gpu(csrMat* Acpu, csrMat* Agpu) {
+  const int n = Acpu->nrows, nnz = Acpu->ia[n];
+  Agpu->owndata = 1;
+  Agpu->nrows = Acpu->nrows;
+  Agpu->ncols = Acpu->ncols;
+  Agpu->nnz = nnz;
+  Agpu->ia = (int*)_evsl_Malloc_device((n + 1) * sizeof(int));
+  Agpu->ja = (int*)_evsl_Malloc_device((size_t)std::max(nnz, 1) * sizeof(int));
+  Agpu->a = (double*)_evsl_Malloc_device((size_t)std::max(nnz, 1) * sizeof(double));
+  evsl_memcpy_host_to_device(Agpu->ia, Acpu->ia, (n + 1) * sizeof(int));
+  evsl_memcpy_host_to_device(Agpu->ja, Acpu->ja, (size_t)nnz * sizeof(int));
+  evsl_memcpy_host_to_device(Agpu->a, Acpu->a, (size_t)nnz * sizeof(double));
+}
+
+void evsl_free_csr_gpu(csrMat* csr) {
+  Context& c = ctx();
+  if (c.op == 1 && c.A.ia == csr->ia) {  // drop the registration first
+    c.A.release();
+    c.op = 0;
+  }
+  _evsl_Free_device(csr->ia);
+  _evsl_Free_device(csr->ja);
+  _evsl_Free_device(csr->a);
+  csr->ia = csr->ja = nullptr;
+  csr->a = nullptr;
+}
+
+// user operator: func(x, y, data) is called with device vectors
+int SetAMatvec(int n, MVFunc func, void* data) {
+  if (int e = ensure_started()) return e;
+  if (n <= 0 || !func) return EVSL_ERR_ARG;
+  Context& c = ctx();
+  c.A.release();
+  c.n = n;
+  c.op = 2;
+  c.func = func;
+  c.func_data = data;
+  return 0;
+}
+
+// generalized problems are outside this library's device path
+int SetBMatrix(csrMat*) { return EVSL_ERR_UNSUPP; }
+int SetBMatvec(int, MVFunc, void*) { return EVSL_ERR_UNSUPP; }
+int SetBSol(SolFuncR, void*) { return EVSL_ERR_UNSUPP; }
+int SetLTSol(SolFuncR, void*) { return EVSL_ERR_UNSUPP; }
+int SetStdEig(void) { return 0; }
+int SetGenEig(void) { return EVSL_ERR_UNSUPP; }
+void SetDiagScal(double*) {}
+
+// ---- RNG (SRC/vect.c:14-61) ---------------------------------------------
+void rand_double(int n, double* v) {
+  const double t = ((double)RAND_MAX) / 2.0;
+  for (int i = 0; i < n; i++) v[i] = (rand() - t) / t;
+}
+
+void randn_double(int n, double* v) {
+  const double two_pi = 2.0 * 3.1415926535;
+  for (int i = 0; i < n; i++) {
+    const double u1 = rand() * (1.0 / RAND_MAX);
+    const double u2 = rand() * (1.0 / RAND_MAX);
+    v[i] = std::sqrt(-2.0 * std::log(u1)) * std::cos(two_pi * u2);
+  }
+}
+
+// host libc stream, then one upload (the reference CUDA build's default
+// generation method, SRC/evsl_cuda.cu:113-116)
+void rand_double_device(int n, double* v) {
+  std::vector<double> h(n);
+  rand_double(n, h.data());
+  evsl_memcpy_host_to_device(v, h.data(), (size_t)n * sizeof(double));
+}
+
+void randn_double_device(int n, double* v) {
+  std::vector<double> h(n);
+  randn_double(n, h.data());
+  evsl_memcpy_host_to_device(v, h.data(), (size_t)n * sizeof(double));
+}
+
+// ---- timing and stats (SRC/timing.c:60-74, SRC/stats.c) --------------------
+// CLOCK_MONOTONIC; like the reference's GPU build it first waits for the device
+double evsl_timer(void) {
+  (void)hipDeviceSynchronize();
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
+void StatsReset(void) { std::memset(&evslstat, 0, sizeof(evslstat)); }
+
+void StatsPrint(FILE* f) {
+  if (!f) return;
+  const evslStat* s = &evslstat;
+  std::fprintf(f, " Timing (sec):\n");
+  if (s->t_iter) std::fprintf(f, "   Iteration time (tot)     :  %f\n", s->t_iter);
+  std::fprintf(f, "   - - - - - - - - - - - - - - - - -\n");
+  if (s->n_polAv)
+    std::fprintf(f, "   Pol(A)*v                 :  %f (%8zu, avg %f)\n", s->t_polAv,
+                 s->n_polAv, s->t_polAv / s->n_polAv);
+  if (s->n_mvA)
+    std::fprintf(f, "   Matvec matrix A          :  %f (%8zu, avg %f)\n", s->t_mvA, s->n_mvA,
+                 s->t_mvA / s->n_mvA);
+  if (s->t_reorth) std::fprintf(f, "   Reorthogonalization      :  %f\n", s->t_reorth);
+  if (s->t_eig) std::fprintf(f, "   LAPACK eig               :  %f\n", s->t_eig);
+  if (s->t_ritz) std::fprintf(f, "   Compute Ritz vectors     :  %f\n", s->t_ritz);
+  std::fflush(f);
+}
+
+// ---- extension: precision, stats, errors --------------------------------
+int evsl_amd_set_filter_precision(int prec) {
+  if (prec != EVSL_PREC_FP64 && prec != EVSL_PREC_MIXED) return EVSL_ERR_ARG;
+  ctx().filter_prec = prec;
+  return 0;
+}
+int evsl_amd_get_filter_precision(void) { return ctx().filter_prec; }
+int evsl_amd_set_refine_passes(int passes) {
+  if (passes < 0 || passes > 8) return EVSL_ERR_ARG;
+  ctx().refine_passes = passes;
+  return 0;
+}
+int evsl_amd_get_stats(evslAmdStats* s) {
+  if (!s) return EVSL_ERR_ARG;
+  *s = ctx().astats;
+  return 0;
+}
+void evsl_amd_reset_stats(void) { std::memset(&ctx().astats, 0, sizeof(evslAmdStats)); }
+const char* evsl_amd_last_error(void) { return last_error(); }
+size_t evsl_amd_stats_size(void) { return sizeof(evslAmdStats); }
+
+}  // extern "C"

@@ -1,0 +1,272 @@
+"""ctypes mirror of the drop-in C interface exported by libevsl_amd.so.
+
+Every call goes through the C ABI declared in include/evsl.h and
+include/evsl_amd.h; this module adds only argument marshalling (numpy in,
+numpy out).  Loading fails loudly when the library has not been built --
+there is no fallback implementation.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .types import MVFUNC, PolParams, c_double_p, c_int_p, dptr, iptr, CsrMat
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libevsl_amd.so")
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+VEC_RAND_DOUBLE = 0
+VEC_RADEMACHER = 1
+VEC_GAUSSIAN = 2
+
+# C entry points declared in include/evsl.h and include/evsl_amd.h
+EXPORTS = [
+    "_evsl_Malloc", "_evsl_Calloc", "_evsl_Realloc", "_evsl_Free",
+    "_evsl_Malloc_device", "_evsl_Calloc_device", "_evsl_Realloc_device", "_evsl_Free_device",
+    "evsl_memcpy_device_to_host", "evsl_memcpy_device_to_device",
+    "evsl_memcpy_host_to_device", "evsl_memset_device",
+    "EVSLStart", "EVSLFinish", "SetAMatrix", "SetAMatvec", "SetBMatrix", "SetBMatvec",
+    "SetBSol", "SetLTSol", "SetStdEig", "SetGenEig", "SetDiagScal",
+    "evsl_create_csr_gpu", "evsl_free_csr_gpu", "SetAMatrix_device_csr",
+    "set_pol_def", "find_pol", "free_pol",
+    "kpmdos", "spslicer", "LanDos", "spslicer2",
+    "ChebLanNr", "ChebLanTr", "LanBounds",
+    "cooMat_to_csrMat", "free_csr", "free_coo", "csr_copy", "arrays_copyto_csrMat",
+    "matvec_csr", "rand_double", "rand_double_device", "randn_double",
+    "randn_double_device", "sort_double", "linspace", "evsl_timer", "StatsPrint",
+    "StatsReset",
+    "evsl_amd_set_filter_precision", "evsl_amd_get_filter_precision",
+    "evsl_amd_set_refine_passes", "evsl_amd_get_stats", "evsl_amd_reset_stats",
+    "evsl_amd_last_error", "evsl_amd_stats_size", "kpmdos_ex", "evsl_amd_bench_filter",
+]
+
+
+class AmdStats(C.Structure):
+    _fields_ = [("t_filter", C.c_double), ("t_reorth", C.c_double), ("t_eig", C.c_double),
+                ("t_ritz", C.c_double), ("n_filter_steps", C.c_size_t),
+                ("n_filter_cols", C.c_size_t), ("n_lanczos", C.c_size_t),
+                ("n_reorth_pass", C.c_size_t), ("filter_bytes", C.c_double),
+                ("reorth_bytes", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def load_library(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    return C.CDLL(path)
+
+
+class EVSL:
+    """The drop-in interface, one process-wide context (as the reference)."""
+
+    def __init__(self, path=LIB_PATH):
+        self.lib = L = load_library(path)
+        D, I, V, VP = C.c_double, C.c_int, None, C.c_void_p
+        dp, ip = c_double_p, c_int_p
+        pp = C.POINTER(PolParams)
+        ddp = C.POINTER(c_double_p)
+        sig = {
+            "_evsl_Malloc_device": (VP, [C.c_size_t]),
+            "_evsl_Free_device": (V, [VP]),
+            "_evsl_Free": (V, [VP]),
+            "evsl_memcpy_device_to_host": (V, [VP, VP, C.c_size_t]),
+            "evsl_memcpy_host_to_device": (V, [VP, VP, C.c_size_t]),
+            "evsl_memcpy_device_to_device": (V, [VP, VP, C.c_size_t]),
+            "evsl_memset_device": (V, [VP, I, C.c_size_t]),
+            "EVSLStart": (I, []),
+            "EVSLFinish": (I, []),
+            "SetAMatrix": (I, [C.POINTER(CsrMat)]),
+            "SetAMatvec": (I, [I, MVFUNC, VP]),
+            "SetAMatrix_device_csr": (I, [C.POINTER(CsrMat)]),
+            "evsl_create_csr_gpu": (V, [C.POINTER(CsrMat), C.POINTER(CsrMat)]),
+            "evsl_free_csr_gpu": (V, [C.POINTER(CsrMat)]),
+            "set_pol_def": (V, [pp]),
+            "find_pol": (I, [dp, pp]),
+            "free_pol": (V, [pp]),
+            "kpmdos": (I, [I, I, I, dp, dp, dp]),
+            "kpmdos_ex": (I, [I, I, I, dp, I, C.c_ulonglong, dp, dp]),
+            "spslicer": (I, [dp, dp, I, dp, I, I]),
+            "spslicer2": (V, [dp, dp, I, I, dp]),
+            "LanDos": (I, [I, I, I, dp, dp, dp, dp]),
+            "ChebLanNr": (I, [dp, I, D, dp, pp, ip, ddp, ddp, ddp, VP]),
+            "ChebLanTr": (I, [I, I, dp, I, D, dp, pp, ip, ddp, ddp, ddp, VP]),
+            "LanBounds": (I, [I, dp, dp, dp]),
+            "rand_double": (V, [I, dp]),
+            "rand_double_device": (V, [I, dp]),
+            "randn_double": (V, [I, dp]),
+            "evsl_timer": (D, []),
+            "StatsReset": (V, []),
+            "evsl_amd_set_filter_precision": (I, [I]),
+            "evsl_amd_get_filter_precision": (I, []),
+            "evsl_amd_set_refine_passes": (I, [I]),
+            "evsl_amd_get_stats": (I, [C.POINTER(AmdStats)]),
+            "evsl_amd_reset_stats": (V, []),
+            "evsl_amd_last_error": (C.c_char_p, []),
+            "evsl_amd_stats_size": (C.c_size_t, []),
+            "evsl_amd_bench_filter": (I, [I, I, I, I, dp, dp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        assert L.evsl_amd_stats_size() == C.sizeof(AmdStats), "evslAmdStats ABI mismatch"
+        self._mat = None
+        self._mv = None
+
+    # ---- lifecycle / operator ----------------------------------------------
+    def start(self):
+        return self.lib.EVSLStart()
+
+    def finish(self):
+        return self.lib.EVSLFinish()
+
+    def set_matrix(self, A):
+        """A: evsl_amd.types.HostCsr."""
+        self._mat = A
+        return self.lib.SetAMatrix(C.byref(A.csr))
+
+    def set_matvec(self, n, func, data=None):
+        self._mv = MVFUNC(func) if not isinstance(func, MVFUNC) else func
+        return self.lib.SetAMatvec(n, self._mv, data)
+
+    def last_error(self):
+        return self.lib.evsl_amd_last_error().decode()
+
+    # ---- device memory -------------------------------------------------------
+    def to_device(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        p = self.lib._evsl_Malloc_device(x.nbytes)
+        if not p:
+            raise MemoryError(self.last_error())
+        self.lib.evsl_memcpy_host_to_device(p, x.ctypes.data, x.nbytes)
+        return p
+
+    def from_device(self, p, count):
+        out = np.empty(count)
+        if count:
+            self.lib.evsl_memcpy_device_to_host(out.ctypes.data, p, out.nbytes)
+        return out
+
+    def free_device(self, p):
+        self.lib._evsl_Free_device(p)
+
+    # ---- filters -----------------------------------------------------------
+    def default_pol(self):
+        pol = PolParams()
+        self.lib.set_pol_def(C.byref(pol))
+        return pol
+
+    def find_pol(self, intv, pol):
+        intv = np.ascontiguousarray(intv, dtype=np.float64)
+        return self.lib.find_pol(dptr(intv), C.byref(pol))
+
+    def free_pol(self, pol):
+        self.lib.free_pol(C.byref(pol))
+
+    # ---- DOS / slicing ---------------------------------------------------------
+    def kpmdos(self, Mdeg, damping, nvec, intv, vec_kind=None, seed=0):
+        intv = np.ascontiguousarray(intv, dtype=np.float64)
+        mu = np.empty(Mdeg + 1)
+        ecnt = C.c_double()
+        if vec_kind is None:
+            err = self.lib.kpmdos(Mdeg, damping, nvec, dptr(intv), dptr(mu), C.byref(ecnt))
+        else:
+            err = self.lib.kpmdos_ex(Mdeg, damping, nvec, dptr(intv), vec_kind, seed,
+                                     dptr(mu), C.byref(ecnt))
+        return err, mu, ecnt.value
+
+    def spslicer(self, mu, intv, n_int, npts):
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        intv = np.ascontiguousarray(intv, dtype=np.float64)
+        sli = np.empty(n_int + 1)
+        err = self.lib.spslicer(dptr(sli), dptr(mu), len(mu) - 1, dptr(intv), n_int, npts)
+        return err, sli
+
+    def spslicer2(self, xi, yi, n_int):
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        yi = np.ascontiguousarray(yi, dtype=np.float64).copy()
+        sli = np.empty(n_int + 1)
+        self.lib.spslicer2(dptr(xi), dptr(yi), n_int, len(xi), dptr(sli))
+        return sli
+
+    def LanDos(self, nvec, msteps, npts, intv):
+        intv = np.ascontiguousarray(intv, dtype=np.float64)
+        x, y, neig = np.empty(npts), np.empty(npts), C.c_double()
+        err = self.lib.LanDos(nvec, msteps, npts, dptr(x), dptr(y), C.byref(neig), dptr(intv))
+        return err, x, y, neig.value
+
+    # ---- solvers -----------------------------------------------------------------
+    def _collect(self, n, nev, lam, W, res):
+        if nev > 0:
+            lam_a = np.ctypeslib.as_array(lam, shape=(nev,)).copy()
+            res_a = np.ctypeslib.as_array(res, shape=(nev,)).copy()
+            Y = self.from_device(C.cast(W, C.c_void_p).value, n * nev).reshape(nev, n).T
+        else:
+            lam_a, res_a, Y = np.empty(0), np.empty(0), np.empty((n, 0))
+        if lam:
+            self.lib._evsl_Free(C.cast(lam, C.c_void_p))
+        if res:
+            self.lib._evsl_Free(C.cast(res, C.c_void_p))
+        if W:
+            self.lib._evsl_Free_device(C.cast(W, C.c_void_p))
+        return lam_a, Y, res_a
+
+    def ChebLanNr(self, intv, maxit, tol, vinit, pol, n=None):
+        """vinit: numpy array (host) or an int device pointer (then give n)."""
+        intv = np.ascontiguousarray(intv, dtype=np.float64)
+        if isinstance(vinit, int):
+            vp = C.cast(vinit, c_double_p)
+            assert n is not None
+        else:
+            vinit = np.ascontiguousarray(vinit, dtype=np.float64)
+            vp = dptr(vinit)
+            n = len(vinit)
+        nev = C.c_int()
+        lam, W, res = c_double_p(), c_double_p(), c_double_p()
+        err = self.lib.ChebLanNr(dptr(intv), maxit, tol, vp, C.byref(pol), C.byref(nev),
+                                 C.byref(lam), C.byref(W), C.byref(res), None)
+        if err:
+            return err, None, None, None
+        return (0,) + self._collect(n, nev.value, lam, W, res)
+
+    def ChebLanTr(self, lanm, nev, intv, maxit, tol, vinit, pol):
+        intv = np.ascontiguousarray(intv, dtype=np.float64)
+        vinit = np.ascontiguousarray(vinit, dtype=np.float64)
+        nev2 = C.c_int()
+        lam, W, res = c_double_p(), c_double_p(), c_double_p()
+        err = self.lib.ChebLanTr(lanm, nev, dptr(intv), maxit, tol, dptr(vinit), C.byref(pol),
+                                 C.byref(nev2), C.byref(lam), C.byref(W), C.byref(res), None)
+        if err:
+            return err, None, None, None
+        return (0,) + self._collect(len(vinit), nev2.value, lam, W, res)
+
+    def LanBounds(self, msteps, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        lmin, lmax = C.c_double(), C.c_double()
+        err = self.lib.LanBounds(msteps, dptr(v), C.byref(lmin), C.byref(lmax))
+        return err, lmin.value, lmax.value
+
+    # ---- extensions -----------------------------------------------------------------
+    def set_precision(self, prec):
+        return self.lib.evsl_amd_set_filter_precision(prec)
+
+    def set_refine_passes(self, k):
+        return self.lib.evsl_amd_set_refine_passes(k)
+
+    def stats(self):
+        s = AmdStats()
+        self.lib.evsl_amd_get_stats(C.byref(s))
+        return s.as_dict()
+
+    def reset_stats(self):
+        self.lib.evsl_amd_reset_stats()
+
+    def bench_filter(self, B, prec, mode, nsteps):
+        ms, by = C.c_double(), C.c_double()
+        err = self.lib.evsl_amd_bench_filter(B, prec, mode, nsteps, C.byref(ms), C.byref(by))
+        return err, ms.value, by.value

@@ -1,0 +1,67 @@
+/*
+ * evsl_amd.h -- extensions of the drop-in interface (include/evsl.h) that
+ * have no counterpart in EVSL 1.1.1.  They expose the MI355X-specific forms
+ * of the same hot path: filter precision, seeded device random vectors for
+ * KPM, all slices of one matrix solved as one batched filtered Lanczos, and
+ * per-call device statistics.
+ */
+#ifndef EVSL_AMD_EXT_H
+#define EVSL_AMD_EXT_H
+
+#include <stddef.h>
+#include "evsl.h"
+
+#define EVSL_PREC_FP64 0  /* filter in FP64 (the reference's arithmetic)      */
+#define EVSL_PREC_MIXED 1 /* FP32 filter, FP64 Lanczos + FP64 Rayleigh-Ritz   */
+
+#define EVSL_VEC_RAND_DOUBLE 0 /* libc rand() U[-1,1], the reference stream */
+#define EVSL_VEC_RADEMACHER 1  /* seeded device counter RNG, +-1            */
+#define EVSL_VEC_GAUSSIAN 2    /* seeded device counter RNG, N(0,1)         */
+
+/* Device-side statistics accumulated since the last reset.  Times are in
+ * seconds measured with HIP events on the library's stream. */
+typedef struct _evslAmdStats {
+  double t_filter;       /* Chebyshev filter (ChebAv / KPM recurrences)   */
+  double t_reorth;       /* CGS/DGKS re-orthogonalization                */
+  double t_eig;          /* tridiagonal eigensolves (host + rotations)    */
+  double t_ritz;         /* Ritz vectors, residuals, Rayleigh-Ritz        */
+  size_t n_filter_steps; /* fused SpMM degree steps launched              */
+  size_t n_filter_cols;  /* sum over those launches of the block width    */
+  size_t n_lanczos;      /* Lanczos steps                                 */
+  size_t n_reorth_pass;  /* Gram-Schmidt passes                           */
+  double filter_bytes;   /* algorithmic bytes of the filter launches      */
+  double reorth_bytes;   /* algorithmic bytes of the Gram-Schmidt passes  */
+} evslAmdStats;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* filter precision for ChebLanNr / ChebLanTr / ChebLanNr_batch */
+int evsl_amd_set_filter_precision(int prec);
+int evsl_amd_get_filter_precision(void);
+/* number of FP64 filtered Rayleigh-Ritz refinement passes in MIXED mode */
+int evsl_amd_set_refine_passes(int passes);
+
+int evsl_amd_get_stats(evslAmdStats *s);
+void evsl_amd_reset_stats(void);
+const char *evsl_amd_last_error(void);
+/* size in bytes of evslAmdStats (ABI check for bindings) */
+size_t evsl_amd_stats_size(void);
+
+/* kpmdos with a choice of random vectors (kind: EVSL_VEC_*); with
+ * EVSL_VEC_RAND_DOUBLE it is kpmdos exactly. */
+int kpmdos_ex(int Mdeg, int damping, int nvec, double *intv, int vec_kind,
+              unsigned long long seed, double *mu, double *ecnt);
+
+/* Measurement hook: time nsteps launches of the fused filter kernel on the
+ * current matrix (block width B in {1,2,4,8,16,32}; mode 1 ChebAv step,
+ * mode 2 KPM step) with HIP events.  Returns the mean launch time and the
+ * algorithmic bytes one launch moves. */
+int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double *ms_per_step,
+                          double *bytes_per_step);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
