@@ -41,11 +41,16 @@ def build_problem(nx, seed):
     return A, lo, hi
 
 
-def run_step(ev, A, lo, hi, args, prec):
-    """One pass of the slicing workload; returns per-slice records."""
-    from evsl_amd.evsl import VEC_RADEMACHER
+def run_step(ev, A, lo, hi, args, form):
+    """One pass of the slicing workload; returns per-slice records.
+
+    form: "plain"   ChebLanNr per slice, FP64 filter (the drop-in call sequence)
+          "batch64" ChebLanNr_batch, FP64 filter (all slices in one blocked run)
+          "mixed"   ChebLanNr_batch, FP32 filter + FP64 Lanczos and refinement
+    """
+    from evsl_amd.evsl import PREC_FP64, PREC_MIXED, VEC_RADEMACHER
     intv = np.array([args.a, args.b, lo, hi])
-    ev.set_precision(prec)
+    ev.set_precision(PREC_MIXED if form == "mixed" else PREC_FP64)
     t0 = time.perf_counter()
     err, mu, ecnt = ev.kpmdos(args.mdeg, 1, args.nvec, intv, vec_kind=VEC_RADEMACHER, seed=7)
     assert err == 0, ev.last_error()
@@ -60,28 +65,43 @@ def run_step(ev, A, lo, hi, args, prec):
     rng = np.random.default_rng(11)
     v0 = rng.uniform(-1.0, 1.0, A.n)
     vd = ev.to_device(v0)
-    slices = []
+    pols, ivs = [], []
     for s in range(args.nslices):
         iv = np.array([sli[s], sli[s + 1], lo, hi])
         pol = ev.default_pol()
         pol.damping, pol.thresh_int, pol.thresh_ext, pol.max_deg = 2, 0.8, 0.2, 3000
         assert ev.find_pol(iv, pol) == 0
-        ev.reset_stats()
-        ts = time.perf_counter()
-        err, lam, Y, res = ev.ChebLanNr(iv, mlan, args.tol, vd, pol, n=A.n)
-        te = time.perf_counter()
+        pols.append(pol)
+        ivs.append(iv)
+    slices = []
+    ev.reset_stats()
+    ts = time.perf_counter()
+    if form == "plain":
+        for s in range(args.nslices):
+            ev.reset_stats()
+            t1 = time.perf_counter()
+            err, lam, Y, res = ev.ChebLanNr(ivs[s], mlan, args.tol, vd, pols[s], n=A.n)
+            assert err == 0, ev.last_error()
+            st = ev.stats()
+            slices.append({"a": float(ivs[s][0]), "b": float(ivs[s][1]), "deg": int(pols[s].deg),
+                           "nev": int(len(lam)), "max_res": float(res.max()) if len(res) else 0.0,
+                           "steps": int(st["n_lanczos"]), "secs": time.perf_counter() - t1,
+                           "lam": lam, "stats": st})
+    else:
+        err, outs = ev.ChebLanNr_batch(ivs, mlan, args.tol, vd, pols, n=A.n)
         assert err == 0, ev.last_error()
-        st = ev.stats()
-        slices.append({"a": float(iv[0]), "b": float(iv[1]), "deg": int(pol.deg),
-                       "nev": int(len(lam)), "max_res": float(res.max()) if len(res) else 0.0,
-                       "steps": int(st["n_lanczos"]), "secs": te - ts,
-                       "t_filter": st["t_filter"], "t_reorth": st["t_reorth"],
-                       "t_ritz": st["t_ritz"], "filter_bytes": st["filter_bytes"],
-                       "reorth_bytes": st["reorth_bytes"], "lam": lam})
-        ev.free_pol(pol)
+        for s, (lam, Y, res) in enumerate(outs):
+            slices.append({"a": float(ivs[s][0]), "b": float(ivs[s][1]), "deg": int(pols[s].deg),
+                           "nev": int(len(lam)),
+                           "max_res": float(res.max()) if len(res) else 0.0, "lam": lam})
+    t_solve = time.perf_counter() - ts
+    stats = ev.stats()
+    for p in pols:
+        ev.free_pol(p)
     ev.free_device(vd)
-    return {"t_kpm": t_kpm, "ecnt": float(ecnt), "sli": sli.tolist(), "mlan": mlan,
-            "slices": slices}
+    return {"t_kpm": t_kpm, "t_solve": t_solve, "ecnt": float(ecnt), "sli": sli.tolist(),
+            "mlan": mlan, "slices": slices, "stats": stats, "cc": 0.5 * (lo + hi),
+            "dd": 0.5 * (hi - lo)}
 
 
 def cpu_baseline(A, lo, hi, step, args):
@@ -117,8 +137,9 @@ def cpu_baseline(A, lo, hi, step, args):
     t_col = (orc.timer() - t0) / k
     del Q
     est = args.mdeg * args.nvec * t_kpm_deg
+    avg_steps = step["stats"]["n_lanczos"] / max(1, len(step["slices"]))
     for s in step["slices"]:
-        steps = s["steps"]
+        steps = s.get("steps", avg_steps)
         est += (steps + 1) * s["deg"] * t_deg
         est += 1.5 * t_col * steps * (steps + 1) / 2.0   # ~1.5 DGKS passes per step
     sample_secs = t_kpm_deg * 20 + t_lan1 + t_col * k
@@ -143,7 +164,9 @@ def main():
     p.add_argument("--mdeg", type=int, default=300)
     p.add_argument("--nvec", type=int, default=40)
     p.add_argument("--tol", type=float, default=1e-8)
-    p.add_argument("--form", choices=["fp64", "mixed"], default="fp64")
+    p.add_argument("--form", choices=["plain", "batch64", "mixed"], default="mixed",
+                   help="timed form (the FP64 plain form is also run once for the ratio)")
+    p.add_argument("--no-plain", action="store_true", help="skip the FP64 plain reference run")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--maxit", type=int, default=0,
                    help="cap the Krylov dimension per slice (short profiling runs only)")
@@ -157,16 +180,17 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
-    from evsl_amd.evsl import EVSL, PREC_FP64, PREC_MIXED
-    prec = PREC_MIXED if args.form == "mixed" else PREC_FP64
+    from evsl_amd.evsl import EVSL, PREC_FP64, PREC_MIXED, VEC_RADEMACHER
     log(f"[bench] building matrix nx={args.nx}")
     A, lo, hi = build_problem(args.nx, seed=2024)
     ev = EVSL()
     assert ev.start() == 0
     assert ev.set_matrix(A) == 0, ev.last_error()
-    from evsl_amd.evsl import VEC_RADEMACHER
+    cc, dd = 0.5 * (lo + hi), 0.5 * (hi - lo)
     if args.filter_only:
-        err, ms, by = ev.bench_filter(1, prec, 1, 50)
+        prec = PREC_MIXED if args.form == "mixed" else PREC_FP64
+        B = args.nslices if args.form != "plain" else 1
+        err, ms, by = ev.bench_filter(B, prec, 1, 50, cc, dd)
         nbytes = 512 << 20
         src, dst = ev.lib._evsl_Malloc_device(nbytes), ev.lib._evsl_Malloc_device(nbytes)
         ev.lib.evsl_memset_device(src, 0, nbytes)
@@ -181,46 +205,78 @@ def main():
     ev.kpmdos(4, 1, 1, np.array([args.a, args.b, lo, hi]), vec_kind=VEC_RADEMACHER, seed=1)
     for w in range(args.warmup):
         log(f"[bench] warmup step {w}")
-        run_step(ev, A, lo, hi, args, prec)
-    log("[bench] timed steps")
+        run_step(ev, A, lo, hi, args, args.form)
+    log(f"[bench] timed steps ({args.form})")
     t0 = time.perf_counter()
     steps = []
     for k in range(args.steps):
-        steps.append(run_step(ev, A, lo, hi, args, prec))
-        log(f"[bench] step {k}: {sum(s['nev'] for s in steps[-1]['slices'])} pairs")
+        steps.append(run_step(ev, A, lo, hi, args, args.form))
+        log(f"[bench] step {k}: {sum(s['nev'] for s in steps[-1]['slices'])} pairs, "
+            f"solve {steps[-1]['t_solve']:.1f}s")
     wall = time.perf_counter() - t0
     pairs = sum(s["nev"] for st in steps for s in st["slices"])
     value = pairs / wall
-
-    # dominant kernel: fused Chebyshev-step SpMM (B = 1) in the run's precision
-    err, ms, by = ev.bench_filter(1, prec, 1, 200)
-    achieved = by / (ms * 1e-3) / 1e9
     last = steps[-1]
-    per_slice = [{k: v for k, v in s.items() if k != "lam"} for s in last["slices"]]
-    filt_bytes = sum(s["filter_bytes"] for s in per_slice)
-    filt_time = sum(s["t_filter"] for s in per_slice)
+
+    plain = None
+    if args.form != "plain" and not args.no_plain:
+        log("[bench] FP64 plain form (reference call sequence), once")
+        tp = time.perf_counter()
+        plain = run_step(ev, A, lo, hi, args, "plain")
+        plain["wall"] = time.perf_counter() - tp
+        log(f"[bench] plain: {sum(s['nev'] for s in plain['slices'])} pairs in {plain['wall']:.1f}s")
+
+    # dominant kernel of the timed form: the fused Chebyshev-step SpMM
+    if args.form == "mixed":
+        kB, kprec, kname = args.nslices, PREC_MIXED, f"k_cheb_spmm<float,{args.nslices},kCheb>"
+    elif args.form == "batch64":
+        kB, kprec, kname = args.nslices, PREC_FP64, f"k_cheb_spmm<double,{args.nslices},kCheb>"
+    else:
+        kB, kprec, kname = 1, PREC_FP64, "k_cheb_spmm<double,1,kCheb>"
+    err, ms, by = ev.bench_filter(kB, kprec, 1, 200, cc, dd)
+    achieved = by / (ms * 1e-3) / 1e9
+    err1, ms1, by1 = ev.bench_filter(1, PREC_FP64, 1, 200, cc, dd)
+    st = last["stats"]
     rec = {
         "metric": METRIC, "value": value, "unit": "eigenpairs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if prec == PREC_FP64 else "f32-filter/f64-lanczos",
+        "dtype": "f64" if args.form != "mixed" else "f32 filter / f64 Lanczos + refinement",
         "data": "synthetic (seeded)",
         "config": {"workload": f"3-D Laplacian {args.nx}^3 + U[0,1e-3] diagonal, "
                                f"[{args.a},{args.b}], KPM {args.mdeg}/{args.nvec} Rademacher, "
                                f"{args.nslices} slices, ChebLanNr tol {args.tol}",
                    "n": A.n, "nnz": A.nnz, "form": args.form, "parallelism": "single GPU"},
         "eigenpairs": pairs // args.steps,
-        "filter_GBps_in_solver": filt_bytes / filt_time / 1e9 if filt_time else None,
-        "per_slice": per_slice, "kpm_s": last["t_kpm"], "ecnt": last["ecnt"],
+        "per_slice_counts": [s["nev"] for s in last["slices"]],
+        "max_residual": max(s["max_res"] for s in last["slices"]),
+        "kpm_s": last["t_kpm"], "solve_s": last["t_solve"],
+        "filter_GBps_in_solver": (st["filter_bytes"] / st["t_filter"] / 1e9
+                                  if args.form != "plain" and st["t_filter"] else None),
+        "breakdown_s": {"filter": st["t_filter"], "reorth": st["t_reorth"],
+                        "ritz_refine": st["t_ritz"]} if args.form != "plain" else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_cheb_spmm<double,1,kCheb>" if prec == PREC_FP64
-                     else "k_cheb_spmm<float,1,kCheb>",
-                     "launch_ms": ms, "bytes_per_launch": by},
+                     "kernel": kname, "launch_ms": ms, "bytes_per_launch": by,
+                     "fp64_b1_GBps": by1 / (ms1 * 1e-3) / 1e9},
     }
+    if plain is not None:
+        pc = [s["nev"] for s in plain["slices"]]
+        pv = sum(pc) / plain["wall"]
+        rec["fp64_plain"] = {"value": pv, "wall_s": plain["wall"], "per_slice_counts": pc,
+                             "max_residual": max(s["max_res"] for s in plain["slices"])}
+        rec["speedup_vs_fp64_plain"] = value / pv
+        rec["counts_match_fp64_plain"] = pc == rec["per_slice_counts"]
+        diffs = []
+        for sp, sm in zip(plain["slices"], last["slices"]):
+            if len(sp["lam"]) == len(sm["lam"]) and len(sp["lam"]):
+                a, b = np.sort(sp["lam"]), np.sort(sm["lam"])
+                diffs.append(float(np.max(np.abs(a - b) / np.abs(a))))
+        rec["max_rel_eig_diff_vs_fp64_plain"] = max(diffs) if diffs else None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline sample")
-        rec["cpu_baseline"] = cpu_baseline(A, lo, hi, last, args)
+        ref = plain if plain is not None else last
+        rec["cpu_baseline"] = cpu_baseline(A, lo, hi, ref, args)
     else:
         rec["cpu_baseline"] = None
     print(json.dumps(rec), flush=True)

@@ -13,11 +13,16 @@
 namespace evsl_dev {
 
 // Row blocks for the LDS-staged CSR kernels: block b covers rows
-// [rb[b], rb[b+1]) and either holds at most kNnzCap nonzeros and kRowsCap
-// rows, or is a single row longer than kNnzCap.
+// [rb[b], rb[b+1]) and either holds at most kNnzCap nonzeros and 256/B rows
+// (one (row, column) lane each for a B-column block), or is a single row
+// longer than kNnzCap.  One table per supported block width.
 constexpr int kWG = 256;
 constexpr int kNnzCap = 2048;
 constexpr int kRowsCap = 256;
+constexpr int kNumWidths = 10;  // B = 1..8, 16, 32
+inline int width_slot(int B) { return B <= 8 ? B - 1 : (B == 16 ? 8 : (B == 32 ? 9 : -1)); }
+inline int width_of_slot(int s) { return s < 8 ? s + 1 : (s == 8 ? 16 : 32); }
+inline int rows_cap_for(int B) { return kWG / B; }
 
 struct Csr {
   int n = 0, nnz = 0;
@@ -25,28 +30,34 @@ struct Csr {
   const int* ja = nullptr;
   const double* a64 = nullptr;
   const float* a32 = nullptr;
-  const int* rb = nullptr;  // row-block starts, nblk + 1 entries
-  int nblk = 0;
+  // per block width: nblk+1 pairs (first row of the block, its row pointer)
+  const int* rb[kNumWidths] = {};
+  int nblk[kNumWidths] = {};
 };
 
-enum Mode { kPlain = 0, kCheb = 1, kKpm = 2 };
+enum Mode { kPlain = 0, kCheb = 1, kKpm = 2, kRes = 3 };
 
-// One Chebyshev degree step over a B-column block (B in {1,2,4,8,16,32}):
+// One Chebyshev degree step over a B-column block (B in {1..8,16,32}; the
+// KPM mode needs B in {1,2,4,8,16,32}):
 //   acc = A * xk
 //   kPlain: xkp1 = acc
 //   kCheb : w = t*(acc - cc*xk) - (xkm1 ? xkm1 : 0); xkp1 = w;
 //           y = (first ? mu0[c]*xk : y) + s[c]*w
 //   kKpm  : w as kCheb; xkp1 = w; part[blk*B + c] = sum_rows w*v0
-// s, mu0: per-column coefficients (device, B entries each).
+//   kRes  : w = t*(acc - cc*xk) - xkm1 + s[c]*v0 (v0 = residual block R);
+//           xkp1 = w; y = (first ? mu0[1]*xk : y) + mu0[0]*w
+// s, mu0: per-column coefficients (device, B entries each; kRes see above).
 // fp32 selects float vectors and matrix values (float accumulation).
 hipError_t cheb_spmm(const Csr& A, int B, bool fp32, Mode mode, const void* xk,
                      const void* xkm1, void* xkp1, void* y, const void* v0,
                      const double* s, const double* mu0, double t, double cc,
                      bool first, double* part, hipStream_t st);
 
-// out[c] = sum_b part[b*B + c] over nblk row blocks (one small launch)
+// out[c] = sum_b part[b*B + c] over nblk row blocks, in a fixed order; uses
+// scratch (>= kSumScratch doubles) for a first level when nblk is large
+constexpr int kSumScratch = 256 * 32;
 hipError_t sum_block_partials(const double* part, int nblk, int B, double* out,
-                              hipStream_t st);
+                              double* scratch, hipStream_t st);
 
 // Fixed-grid reductions: partial buffers hold kRedBlocks entries.
 constexpr int kRedBlocks = 1024;
@@ -71,6 +82,9 @@ hipError_t scal_copy(size_t n, double a, const double* x, double* y, hipStream_t
 // x[i] = v
 hipError_t fill(size_t n, double v, double* x, hipStream_t st);
 hipError_t fill_u32(size_t n, unsigned v, unsigned* x, hipStream_t st);
+// FP32 copy of (A - cc I) * scale (diagonal entries shifted); rows must
+// hold a stored diagonal entry
+hipError_t shift_scale_f32(const Csr& A, double cc, double scale, float* out, hipStream_t st);
 // Z (k x k, column-major) = identity
 hipError_t set_identity(int k, double* Z, hipStream_t st);
 // precision conversions
@@ -104,6 +118,12 @@ hipError_t col_res2(size_t n, int m, const double* W, size_t ldw, const double* 
 // X(:,j) *= f(s[j]) with f = 1/sqrt(s) (normalise by squared norms)
 hipError_t col_scale_rsqrt(size_t n, int m, double* X, size_t ldx, const double* s2,
                            hipStream_t st);
+// U(:,j) = p[j] * U(:,j) + Y(:,j)
+hipError_t col_scale_add(size_t n, int m, double* U, size_t ldu, const double* p, const double* Y,
+                         size_t ldy, hipStream_t st);
+// R(:,j) = (W(:,j) - th[j] * U(:,j)) * inv_dd
+hipError_t col_resid(size_t n, int m, const double* W, size_t ldw, const double* U, size_t ldu,
+                     const double* th, double inv_dd, double* R, size_t ldr, hipStream_t st);
 // dst(:,j) = src(:, idx[j]) for j < m  (rows nrow)
 hipError_t gather_cols(size_t nrow, int m, const double* src, size_t lds, const int* idx,
                        double* dst, size_t ldd, hipStream_t st);
@@ -125,5 +145,23 @@ hipError_t rm_col_scale_rsqrt(size_t n, int B, double* X, const double* s2,
 // x(r*B+c) for r<n, c<B from Rademacher / Gaussian streams keyed by (seed, col0+c, r)
 hipError_t rm_random(size_t n, int B, int kind, unsigned long long seed, int col0,
                      int ncols, double* X, hipStream_t st);
+
+// Interleave B column vectors (device pointers cols[0..B)) into a row-major
+// block X (r*B + c), converting to float if fp32; and the inverse (scaled by
+// scale[c] if scale != nullptr).
+hipError_t pack_cols(size_t n, int B, const double* const* cols, void* X, bool fp32,
+                     hipStream_t st);
+hipError_t unpack_cols(size_t n, int B, const void* X, bool fp32, double* const* cols,
+                       hipStream_t st);
+// row-major block <-> column-major panel (ld) of B columns starting at col0
+hipError_t rm_to_cm(size_t n, int B, const double* X, double* P, size_t ld, hipStream_t st);
+hipError_t cm_to_rm(size_t n, int B, const double* P, size_t ld, double* X, hipStream_t st);
+
+// C(m x p) = A(n x m)^T * B(n x p), all column-major FP64; tall-skinny
+// (split over rows, partials summed by a second launch).  part must hold
+// gemm_tn_part_elems(n, m, p) doubles.
+size_t gemm_tn_part_elems(size_t n, int m, int p);
+hipError_t gemm_tn(size_t n, int m, int p, const double* A, size_t lda, const double* B,
+                   size_t ldb, double* C, size_t ldc, double* part, hipStream_t st);
 
 }  // namespace evsl_dev

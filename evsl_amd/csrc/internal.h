@@ -49,17 +49,25 @@ struct DevBuf {
 // ---- operator ----------------------------------------------------------
 struct DevMatrix {
   int n = 0, nnz = 0;
-  int *ia = nullptr, *ja = nullptr, *rb = nullptr;
+  int *ia = nullptr, *ja = nullptr;
   double* a64 = nullptr;
-  float* a32 = nullptr;
-  int nblk = 0;
+  float* a32 = nullptr;            // FP32 copy of the values (unshifted)
+  float* a32s = nullptr;           // FP32 copy of (A - shift_cc I) * shift_scale
+  double shift_cc = 0.0, shift_scale = 0.0;
+  bool full_diag = false;          // every row stores its diagonal entry
+  int* rb[evsl_dev::kNumWidths] = {};
+  int nblk[evsl_dev::kNumWidths] = {};
   bool owns_csr = false;  // ia/ja/a64 allocated here (SetAMatrix)
   evsl_dev::Csr view() const;
   void release();
 };
 
-// Build the LDS row-block partition from host row pointers.
-std::vector<int> make_row_blocks(const int* ia_h, int n);
+// Build the LDS row-block partition (at most rows_cap rows and kNnzCap
+// nonzeros per block) from host row pointers.
+std::vector<int> make_row_blocks(const int* ia_h, int n, int rows_cap);
+// make sure the shifted FP32 copy for (cc, scale) exists; false if the
+// matrix lacks stored diagonal entries
+int ensure_a32_shifted(double cc, double scale, bool* ok);
 
 struct Context {
   bool started = false;
@@ -71,6 +79,7 @@ struct Context {
   hipStream_t stream = nullptr;  // the legacy null stream (serialises with user code)
   int filter_prec = EVSL_PREC_FP64;
   int refine_passes = 1;
+  double refine_guard = 0.8;
   // scratch for reductions
   DevBuf red_part, red_scalars;
   // pinned host scalars for device->host reads
@@ -87,6 +96,8 @@ int apply_A(const double* x, double* y);
 // device scratch for reductions (>= kRedBlocks * 64 doubles, plus scalars)
 double* red_part();
 double* red_scalar(int i);
+// kSumScratch doubles at the end of the reduction scratch (two-level sums)
+double* red_scratch();
 // copy cnt doubles from device to the pinned host buffer and wait
 int read_scalars(const double* d, int cnt, double* out);
 
@@ -124,6 +135,17 @@ int tridiag_rotations_to_device(int kdim, const RotationLog& rec, const std::vec
 // ---- polynomial helpers (filters.cpp) ----------------------------------
 int dampcf(int m, int damping, double* jac);
 int chebxPltd(int m, const double* mu, int npts, const double* xi, double* yi);
+
+// Operands of one filter application y = p((A - cc)/dd) x.  In FP32 the
+// matrix is stored pre-shifted and scaled by a power of two,
+// (A - cc I) * 2^-s with 2^s ~ dd, so that the shift is exact in FP64 before
+// rounding and the epilogue needs no cc*x term (cc = 0, t scaled by 2^s).
+struct FilterOp {
+  evsl_dev::Csr A;
+  double t1 = 0.0, t2 = 0.0, cc = 0.0;
+  bool f32 = false;
+};
+int make_filter_op(double cc, double dd, bool f32, FilterOp* op);
 
 // ---- filter application (lanczos.cpp) -----------------------------------
 struct ChebWork {

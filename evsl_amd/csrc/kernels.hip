@@ -51,7 +51,7 @@ __device__ __forceinline__ double block_sum(double v, double* sh /*>=4*/) {
 template <typename VT, int B, int MODE>
 __global__ __launch_bounds__(kWG) void k_cheb_spmm(
     const int* __restrict__ ia, const int* __restrict__ ja, const VT* __restrict__ av,
-    const int* __restrict__ rb, int nblk, const VT* __restrict__ xk,
+    const int2* __restrict__ rbp, int nblk, const VT* __restrict__ xk,
     const VT* __restrict__ xkm1, VT* __restrict__ xkp1, VT* __restrict__ y,
     const VT* __restrict__ v0, const double* __restrict__ scol,
     const double* __restrict__ mu0col, double t_, double cc_, int first,
@@ -59,65 +59,122 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
   __shared__ int s_ja[kNnzCap];
   __shared__ VT s_a[kNnzCap];
   __shared__ double s_red[kWG];
-  constexpr int NG = kWG / B;
+  constexpr int NG = kWG / B;  // row groups; lanes >= NG*B idle when B is not a power of 2
   const int blk = xcd_remap(blockIdx.x, nblk);
-  const int r0 = rb[blk], r1 = rb[blk + 1];
-  const int p0 = ia[r0], p1 = ia[r1];
+  // block table entries carry (first row, its row pointer): one load for both ends
+  const int2 e0 = rbp[blk], e1 = rbp[blk + 1];
+  const int r0 = e0.x, r1 = e1.x, p0 = e0.y, p1 = e1.y;
   const int nz = p1 - p0;
-  const int col = threadIdx.x % B, grp = threadIdx.x / B;
+  const int col = threadIdx.x % B;
+  const bool act = threadIdx.x < NG * B;
+  const int grp = threadIdx.x / B;
   const VT t = (VT)t_, cc = (VT)cc_;
-  VT s = 0, mu0 = 0;
+  // kRes: v0 holds the residual block R and scol the per-column factors
+  // 2*T_k(theta_c); mu0col[0..1] = (mu_k, mu_{k-1}) shared by all columns
+  VT s = 0, mu0 = 0, g = 0;
   if (MODE == kCheb) {
     s = (VT)scol[col];
     if (first) mu0 = (VT)mu0col[col];
+  } else if (MODE == kRes) {
+    g = (VT)scol[col];
+    s = (VT)mu0col[0];
+    if (first) mu0 = (VT)mu0col[1];
   }
+  const bool need_xr = first || MODE == kKpm || cc_ != 0.0;
   double dacc = 0.0;
 
-  auto epilogue = [&](int r, VT acc) {
-    const size_t o = (size_t)r * B + col;
+  // epilogue operands of a row, loaded ahead of the gathers
+  struct Ops {
+    VT xr, km1, yo, rr, v;
+  };
+  auto prefetch = [&](int r) {
+    Ops o{};
+    const size_t i = (size_t)r * B + col;
+    if (MODE == kPlain) return o;
+    o.xr = need_xr ? xk[i] : (VT)0;
+    o.km1 = xkm1 ? xkm1[i] : (VT)0;
+    if ((MODE == kCheb || MODE == kRes) && !first) o.yo = y[i];
+    if (MODE == kRes) o.rr = v0[i];
+    if (MODE == kKpm) o.v = v0[i];
+    return o;
+  };
+  auto epilogue = [&](int r, VT acc, const Ops& o) {
+    const size_t i = (size_t)r * B + col;
     if (MODE == kPlain) {
-      xkp1[o] = acc;
+      xkp1[i] = acc;
       return;
     }
-    const VT xr = xk[o];
-    VT w = t * (acc - cc * xr);
-    if (xkm1) w -= xkm1[o];
-    xkp1[o] = w;
-    if (MODE == kCheb) {
-      const VT yo = first ? mu0 * xr : y[o];
-      y[o] = yo + s * w;
+    VT w = t * (acc - cc * o.xr) - o.km1;
+    if (MODE == kRes) w += g * o.rr;
+    xkp1[i] = w;
+    if (MODE == kCheb || MODE == kRes) {
+      const VT yo = first ? mu0 * o.xr : o.yo;
+      y[i] = yo + s * w;
     } else {
-      dacc += (double)w * (double)v0[o];
+      dacc += (double)w * (double)o.v;
     }
   };
 
   if (nz <= kNnzCap) {
+    // row pointers and epilogue operands of this lane's first row go out
+    // with the staging loads
+    const int rf = r0 + grp;
+    int qf0 = 0, qf1 = 0;
+    Ops of{};
+    if (act && rf < r1) {
+      qf0 = ia[rf] - p0;
+      qf1 = ia[rf + 1] - p0;
+      of = prefetch(rf);
+    }
     for (int p = threadIdx.x; p < nz; p += kWG) {
       s_ja[p] = ja[p0 + p];
       s_a[p] = av[p0 + p];
     }
     __syncthreads();
-    for (int r = r0 + grp; r < r1; r += NG) {
-      const int q0 = ia[r] - p0, q1 = ia[r + 1] - p0;
+    for (int r = rf; act && r < r1; r += NG) {
+      int q0 = qf0, q1 = qf1;
+      Ops o = of;
+      if (r != rf) {
+        q0 = ia[r] - p0;
+        q1 = ia[r + 1] - p0;
+        o = prefetch(r);
+      }
       VT acc = 0;
-      for (int q = q0; q < q1; ++q) acc += s_a[q] * xk[(size_t)s_ja[q] * B + col];
-      epilogue(r, acc);
+      // eight independent gathers in flight per lane; zero padding leaves the
+      // sequential summation order (and its rounding) unchanged
+      for (int q = q0; q < q1; q += 8) {
+        int jj[8];
+        VT aa[8], xx[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const bool ok = q + u < q1;
+          jj[u] = ok ? s_ja[q + u] : 0;
+          aa[u] = ok ? s_a[q + u] : (VT)0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xx[u] = (q + u < q1) ? xk[(size_t)jj[u] * B + col] : (VT)0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += aa[u] * xx[u];
+      }
+      epilogue(r, acc, o);
     }
   } else {
     // one long row: all groups split its nonzeros, combine through LDS
     VT acc = 0;
-    for (int p = p0 + grp; p < p1; p += NG) acc += av[p] * xk[(size_t)ja[p] * B + col];
+    for (int p = p0 + grp; act && p < p1; p += NG) acc += av[p] * xk[(size_t)ja[p] * B + col];
     s_red[threadIdx.x] = (double)acc;
     __syncthreads();
-    for (int st = NG / 2; st > 0; st >>= 1) {
-      if (grp < st) s_red[threadIdx.x] += s_red[threadIdx.x + st * B];
-      __syncthreads();
+    if (grp == 0) {
+      double tot = 0.0;
+#pragma unroll 1
+      for (int gg = 0; gg < NG; ++gg) tot += s_red[gg * B + col];
+      epilogue(r0, (VT)tot, prefetch(r0));
     }
-    if (grp == 0) epilogue(r0, (VT)s_red[col]);
     __syncthreads();
   }
 
-  if (MODE == kKpm) {
+  if (MODE == kKpm) {  // B is a power of two here (NG*B == kWG)
+    static_assert(MODE != kKpm || (kWG % B) == 0, "KPM blocks need B | 256");
     s_red[threadIdx.x] = dacc;
     __syncthreads();
     for (int st = NG / 2; st > 0; st >>= 1) {
@@ -128,21 +185,28 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
   }
 }
 
-// out[c] = sum_b part[b*B + c]
+// out[c] = sum over blocks b in [b0, b1) of part[b*B + c]; grid-strided
+// over block ranges when called with gridDim.x > 1 (then out = scratch of
+// gridDim.x * B values)
 __global__ __launch_bounds__(kWG) void k_sum_block_partials(const double* __restrict__ part,
                                                             int nblk, int B,
                                                             double* __restrict__ out) {
   __shared__ double s_red[kWG];
-  const int col = threadIdx.x % B, grp = threadIdx.x / B, NG = kWG / B;
+  const int NG = kWG / B;
+  const int col = threadIdx.x % B, grp = threadIdx.x / B;
+  const bool act = (int)threadIdx.x < NG * B;
+  const int per = (nblk + gridDim.x - 1) / gridDim.x;
+  const int b0 = blockIdx.x * per, b1 = min(nblk, b0 + per);
   double a = 0.0;
-  for (int b = grp; b < nblk; b += NG) a += part[(size_t)b * B + col];
+  if (act)
+    for (int b = b0 + grp; b < b1; b += NG) a += part[(size_t)b * B + col];
   s_red[threadIdx.x] = a;
   __syncthreads();
-  for (int st = NG / 2; st > 0; st >>= 1) {
-    if (grp < st) s_red[threadIdx.x] += s_red[threadIdx.x + st * B];
-    __syncthreads();
+  if (act && grp == 0) {
+    double t = 0.0;
+    for (int g = 0; g < NG; ++g) t += s_red[g * B + col];
+    out[(size_t)blockIdx.x * B + col] = t;
   }
-  if (grp == 0) out[col] = s_red[col];
 }
 
 // ---------------------------------------------------------------------
@@ -449,6 +513,34 @@ __global__ void k_col_scale_rsqrt(size_t n, double* __restrict__ X, size_t ldx,
     x[r] *= s;
 }
 
+// U(:,j) = p[j] * U(:,j) + Y(:,j)
+__global__ void k_col_scale_add(size_t n, double* __restrict__ U, size_t ldu,
+                                const double* __restrict__ p, const double* __restrict__ Y,
+                                size_t ldy) {
+  const int j = blockIdx.y;
+  const double pj = p[j];
+  double* u = U + (size_t)j * ldu;
+  const double* yv = Y + (size_t)j * ldy;
+  for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (size_t)gridDim.x * blockDim.x)
+    u[r] = pj * u[r] + yv[r];
+}
+
+// R(:,j) = (W(:,j) - th[j] * U(:,j)) * inv_dd
+__global__ void k_col_resid(size_t n, const double* __restrict__ W, size_t ldw,
+                            const double* __restrict__ U, size_t ldu,
+                            const double* __restrict__ th, double inv_dd,
+                            double* __restrict__ R, size_t ldr) {
+  const int j = blockIdx.y;
+  const double tj = th[j];
+  const double* w = W + (size_t)j * ldw;
+  const double* u = U + (size_t)j * ldu;
+  double* rr = R + (size_t)j * ldr;
+  for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (size_t)gridDim.x * blockDim.x)
+    rr[r] = (w[r] - tj * u[r]) * inv_dd;
+}
+
 __global__ void k_gather_cols(size_t nrow, const double* __restrict__ src, size_t lds,
                               const int* __restrict__ idx, double* __restrict__ dst,
                               size_t ldd) {
@@ -577,6 +669,123 @@ __global__ void k_rm_random(size_t n, int B, int kind, unsigned long long seed, 
   }
 }
 
+template <typename T>
+__global__ void k_pack_cols(size_t n, int B, const double* const* __restrict__ cols,
+                            T* __restrict__ X) {
+  const size_t tot = n * (size_t)B;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % B);
+    X[i] = (T)cols[c][i / B];
+  }
+}
+
+template <typename T>
+__global__ void k_unpack_cols(size_t n, int B, const T* __restrict__ X,
+                              double* const* __restrict__ cols) {
+  const size_t tot = n * (size_t)B;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % B);
+    cols[c][i / B] = (double)X[i];
+  }
+}
+
+template <bool TO_CM>
+__global__ void k_rm_cm(size_t n, int B, const double* __restrict__ src,
+                        double* __restrict__ dst, size_t ld) {
+  const size_t tot = n * (size_t)B;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % B);
+    const size_t r = i / B;
+    if (TO_CM)
+      dst[(size_t)c * ld + r] = src[i];
+    else
+      dst[i] = src[(size_t)c * ld + r];
+  }
+}
+
+// Gram panel: part[s][j][i] = sum over rows of chunk s of A(r,i) * B(r,j)
+constexpr int kGnRows = 8192;  // rows per split
+__global__ __launch_bounds__(kWG) void k_gemm_tn_partial(size_t n, int m, int p,
+                                                         const double* __restrict__ A,
+                                                         size_t lda,
+                                                         const double* __restrict__ Bm,
+                                                         size_t ldb,
+                                                         double* __restrict__ part) {
+  __shared__ double As[kTK][kTM + 1];
+  __shared__ double Bs[kTK][kTN + 1];
+  const int i0 = blockIdx.x * kTM, j0 = blockIdx.y * kTN;
+  const size_t rbeg = (size_t)blockIdx.z * kGnRows;
+  const size_t rend = min(n, rbeg + kGnRows);
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  double c[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c[i][j] = 0.0;
+  for (size_t r0 = rbeg; r0 < rend; r0 += kTK) {
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const int e = threadIdx.x + l * kWG;
+      const int rr = e % kTK, cc = e / kTK;  // 16 rows x 64 columns
+      const size_t gr = r0 + rr;
+      const int gi = i0 + cc, gj = j0 + cc;
+      As[rr][cc] = (gr < rend && gi < m) ? A[(size_t)gi * lda + gr] : 0.0;
+      Bs[rr][cc] = (gr < rend && gj < p) ? Bm[(size_t)gj * ldb + gr] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kTK; ++kk) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][tx + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][ty + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) c[i][j] += a[i] * b[j];
+    }
+    __syncthreads();
+  }
+  double* out = part + (size_t)blockIdx.z * m * p;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int gj = j0 + ty + 16 * j;
+    if (gj >= p) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int gi = i0 + tx + 16 * i;
+      if (gi < m) out[(size_t)gj * m + gi] = c[i][j];
+    }
+  }
+}
+
+__global__ void k_sum_splits(int nsplit, size_t cnt, const double* __restrict__ part,
+                             int m, double* __restrict__ C, size_t ldc) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < cnt;
+       e += (size_t)gridDim.x * blockDim.x) {
+    double a = 0.0;
+    for (int s = 0; s < nsplit; ++s) a += part[(size_t)s * cnt + e];
+    const size_t i = e % m, j = e / m;
+    C[j * ldc + i] = a;
+  }
+}
+
+// out[k] = (a[k] - (ja[k] == row ? cc : 0)) * scale, one thread per row
+__global__ void k_shift_scale(int n, const int* __restrict__ ia, const int* __restrict__ ja,
+                              const double* __restrict__ a, double cc, double scale,
+                              float* __restrict__ out) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    for (int k = ia[r]; k < ia[r + 1]; ++k) {
+      const double v = (ja[k] == r) ? a[k] - cc : a[k];
+      out[k] = (float)(v * scale);
+    }
+  }
+}
+
 inline unsigned grid_for(size_t n, int per_block, unsigned cap) {
   size_t g = (n + per_block - 1) / per_block;
   if (g < 1) g = 1;
@@ -594,22 +803,43 @@ hipError_t launch_cheb_B(const Csr& A, int B, const void* xk, const void* xkm1, 
   else
     av = A.a64;
   if (!av) return hipErrorInvalidValue;
-  dim3 grid(A.nblk), block(kWG);
+  const int slot = width_slot(B);
+  if (slot < 0 || !A.rb[slot]) return hipErrorInvalidValue;
+  const int nblk = A.nblk[slot];
+  const int2* rb = reinterpret_cast<const int2*>(A.rb[slot]);
+  dim3 grid(nblk), block(kWG);
 #define EVSL_CHEB_CASE(BB)                                                                   \
   case BB:                                                                                   \
     hipLaunchKernelGGL((k_cheb_spmm<VT, BB, MODE>), grid, block, 0, st, A.ia, A.ja, av,      \
-                       A.rb, A.nblk, (const VT*)xk, (const VT*)xkm1, (VT*)xkp1, (VT*)y,      \
+                       rb, nblk, (const VT*)xk, (const VT*)xkm1, (VT*)xkp1, (VT*)y,      \
                        (const VT*)v0, s, mu0, t, cc, first ? 1 : 0, part);                   \
     break;
-  switch (B) {
-    EVSL_CHEB_CASE(1)
-    EVSL_CHEB_CASE(2)
-    EVSL_CHEB_CASE(4)
-    EVSL_CHEB_CASE(8)
-    EVSL_CHEB_CASE(16)
-    EVSL_CHEB_CASE(32)
-    default:
-      return hipErrorInvalidValue;
+  if constexpr (MODE == kKpm) {
+    switch (B) {
+      EVSL_CHEB_CASE(1)
+      EVSL_CHEB_CASE(2)
+      EVSL_CHEB_CASE(4)
+      EVSL_CHEB_CASE(8)
+      EVSL_CHEB_CASE(16)
+      EVSL_CHEB_CASE(32)
+      default:
+        return hipErrorInvalidValue;
+    }
+  } else {
+    switch (B) {
+      EVSL_CHEB_CASE(1)
+      EVSL_CHEB_CASE(2)
+      EVSL_CHEB_CASE(3)
+      EVSL_CHEB_CASE(4)
+      EVSL_CHEB_CASE(5)
+      EVSL_CHEB_CASE(6)
+      EVSL_CHEB_CASE(7)
+      EVSL_CHEB_CASE(8)
+      EVSL_CHEB_CASE(16)
+      EVSL_CHEB_CASE(32)
+      default:
+        return hipErrorInvalidValue;
+    }
   }
 #undef EVSL_CHEB_CASE
   return hipGetLastError();
@@ -624,7 +854,7 @@ hipError_t cheb_spmm(const Csr& A, int B, bool fp32, Mode mode, const void* xk,
                      const void* xkm1, void* xkp1, void* y, const void* v0, const double* s,
                      const double* mu0, double t, double cc, bool first, double* part,
                      hipStream_t st) {
-  if (A.nblk <= 0) return hipSuccess;
+  if (A.n <= 0) return hipSuccess;
   if (fp32) {
     switch (mode) {
       case kPlain:
@@ -633,6 +863,8 @@ hipError_t cheb_spmm(const Csr& A, int B, bool fp32, Mode mode, const void* xk,
         return launch_cheb_B<float, kCheb>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
       case kKpm:
         return launch_cheb_B<float, kKpm>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
+      case kRes:
+        return launch_cheb_B<float, kRes>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
     }
   } else {
     switch (mode) {
@@ -642,15 +874,24 @@ hipError_t cheb_spmm(const Csr& A, int B, bool fp32, Mode mode, const void* xk,
         return launch_cheb_B<double, kCheb>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
       case kKpm:
         return launch_cheb_B<double, kKpm>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
+      case kRes:
+        return launch_cheb_B<double, kRes>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
     }
   }
   return hipErrorInvalidValue;
 }
 
 hipError_t sum_block_partials(const double* part, int nblk, int B, double* out,
-                              hipStream_t st) {
-  if (B < 1 || B > kWG || (kWG % B) != 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_sum_block_partials, dim3(1), dim3(kWG), 0, st, part, nblk, B, out);
+                              double* scratch, hipStream_t st) {
+  if (B < 1 || B > 32) return hipErrorInvalidValue;
+  const int G = kSumScratch / 32;  // first-level blocks (<= 256)
+  if ((size_t)nblk * B <= 16384 || !scratch) {
+    hipLaunchKernelGGL(k_sum_block_partials, dim3(1), dim3(kWG), 0, st, part, nblk, B, out);
+  } else {
+    hipLaunchKernelGGL(k_sum_block_partials, dim3(G), dim3(kWG), 0, st, part, nblk, B, scratch);
+    hipLaunchKernelGGL(k_sum_block_partials, dim3(1), dim3(kWG), 0, st,
+                       (const double*)scratch, G, B, out);
+  }
   return hipGetLastError();
 }
 
@@ -702,6 +943,12 @@ hipError_t fill(size_t n, double v, double* x, hipStream_t st) {
 
 hipError_t fill_u32(size_t n, unsigned v, unsigned* x, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, kWG, 4096)), dim3(kWG), 0, st, n, v, x);
+  return hipGetLastError();
+}
+
+hipError_t shift_scale_f32(const Csr& A, double cc, double scale, float* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_shift_scale, dim3(grid_for(A.n, kWG, 4096)), dim3(kWG), 0, st, A.n, A.ia,
+                     A.ja, A.a64, cc, scale, out);
   return hipGetLastError();
 }
 
@@ -786,6 +1033,22 @@ hipError_t col_scale_rsqrt(size_t n, int m, double* X, size_t ldx, const double*
   return hipGetLastError();
 }
 
+hipError_t col_scale_add(size_t n, int m, double* U, size_t ldu, const double* p, const double* Y,
+                         size_t ldy, hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_col_scale_add, dim3(grid_for(n, kWG, 256), m), dim3(kWG), 0, st, n, U,
+                     ldu, p, Y, ldy);
+  return hipGetLastError();
+}
+
+hipError_t col_resid(size_t n, int m, const double* W, size_t ldw, const double* U, size_t ldu,
+                     const double* th, double inv_dd, double* R, size_t ldr, hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_col_resid, dim3(grid_for(n, kWG, 256), m), dim3(kWG), 0, st, n, W, ldw,
+                     U, ldu, th, inv_dd, R, ldr);
+  return hipGetLastError();
+}
+
 hipError_t gather_cols(size_t nrow, int m, const double* src, size_t lds, const int* idx,
                        double* dst, size_t ldd, hipStream_t st) {
   if (m <= 0) return hipSuccess;
@@ -808,7 +1071,8 @@ hipError_t rm_col_nrm2(size_t n, int B, const double* X, double* part, double* o
   const int NG = kWG / B;
   const unsigned g = grid_for(n, NG, 1024);
   hipLaunchKernelGGL(k_rm_col_nrm2, dim3(g), dim3(kWG), 0, st, n, B, X, part);
-  hipLaunchKernelGGL(k_sum_block_partials, dim3(1), dim3(kWG), 0, st, part, (int)g, B, out);
+  hipLaunchKernelGGL(k_sum_block_partials, dim3(1), dim3(kWG), 0, st, (const double*)part,
+                     (int)g, B, out);
   return hipGetLastError();
 }
 
@@ -822,6 +1086,57 @@ hipError_t rm_random(size_t n, int B, int kind, unsigned long long seed, int col
                      double* X, hipStream_t st) {
   hipLaunchKernelGGL(k_rm_random, dim3(grid_for(n * B, kWG, 8192)), dim3(kWG), 0, st, n, B,
                      kind, seed, col0, ncols, X);
+  return hipGetLastError();
+}
+
+hipError_t pack_cols(size_t n, int B, const double* const* cols, void* X, bool fp32,
+                     hipStream_t st) {
+  const unsigned g = grid_for(n * B, kWG, 8192);
+  if (fp32)
+    hipLaunchKernelGGL(k_pack_cols<float>, dim3(g), dim3(kWG), 0, st, n, B, cols, (float*)X);
+  else
+    hipLaunchKernelGGL(k_pack_cols<double>, dim3(g), dim3(kWG), 0, st, n, B, cols, (double*)X);
+  return hipGetLastError();
+}
+
+hipError_t unpack_cols(size_t n, int B, const void* X, bool fp32, double* const* cols,
+                       hipStream_t st) {
+  const unsigned g = grid_for(n * B, kWG, 8192);
+  if (fp32)
+    hipLaunchKernelGGL(k_unpack_cols<float>, dim3(g), dim3(kWG), 0, st, n, B, (const float*)X,
+                       cols);
+  else
+    hipLaunchKernelGGL(k_unpack_cols<double>, dim3(g), dim3(kWG), 0, st, n, B,
+                       (const double*)X, cols);
+  return hipGetLastError();
+}
+
+hipError_t rm_to_cm(size_t n, int B, const double* X, double* P, size_t ld, hipStream_t st) {
+  hipLaunchKernelGGL(k_rm_cm<true>, dim3(grid_for(n * B, kWG, 8192)), dim3(kWG), 0, st, n, B,
+                     X, P, ld);
+  return hipGetLastError();
+}
+
+hipError_t cm_to_rm(size_t n, int B, const double* P, size_t ld, double* X, hipStream_t st) {
+  hipLaunchKernelGGL(k_rm_cm<false>, dim3(grid_for(n * B, kWG, 8192)), dim3(kWG), 0, st, n, B,
+                     P, X, ld);
+  return hipGetLastError();
+}
+
+size_t gemm_tn_part_elems(size_t n, int m, int p) {
+  const size_t nsplit = (n + kGnRows - 1) / kGnRows;
+  return nsplit * (size_t)m * p;
+}
+
+hipError_t gemm_tn(size_t n, int m, int p, const double* A, size_t lda, const double* B,
+                   size_t ldb, double* C, size_t ldc, double* part, hipStream_t st) {
+  if (m <= 0 || p <= 0) return hipSuccess;
+  const int nsplit = (int)((n + kGnRows - 1) / kGnRows);
+  dim3 grid((m + kTM - 1) / kTM, (p + kTN - 1) / kTN, nsplit);
+  hipLaunchKernelGGL(k_gemm_tn_partial, grid, dim3(kWG), 0, st, n, m, p, A, lda, B, ldb, part);
+  const size_t cnt = (size_t)m * p;
+  hipLaunchKernelGGL(k_sum_splits, dim3(grid_for(cnt, kWG, 4096)), dim3(kWG), 0, st, nsplit, cnt,
+                     part, m, C, ldc);
   return hipGetLastError();
 }
 

@@ -76,9 +76,9 @@ int kpm_impl(int Mdeg, int damping, int nvec, const double* intv, int kind,
         const void* xkm1 = (k == 0) ? nullptr : (k == 1 ? x0 : buf[(k - 2) % 3]);
         EVSL_HIP(evsl_dev::cheb_spmm(A, B, false, evsl_dev::kKpm, xk, xkm1, xkp1, nullptr, x0,
                                      nullptr, nullptr, sc, ctr, false, red_part(), c.stream));
-        EVSL_HIP(evsl_dev::sum_block_partials(red_part(), A.nblk, B,
+        EVSL_HIP(evsl_dev::sum_block_partials(red_part(), A.nblk[evsl_dev::width_slot(B)], B,
                                               dots.as<double>() + (size_t)(k + 1) * B,
-                                              c.stream));
+                                              red_scratch(), c.stream));
         c.astats.filter_bytes += (double)c.A.nnz * 12 + 4.0 * (n + 1) +
                                  (double)(k == 0 ? 3 : 4) * nb * sizeof(double);
       }
@@ -234,15 +234,16 @@ void spslicer2(double* xi, double* yi, int n_int, int npts, double* sli) {
 // Time nsteps fused filter launches (block width B, prec) on the current
 // matrix with HIP events; reports the average launch time and the
 // algorithmic bytes of one launch.  mode 1: ChebAv step, 2: KPM step.
-int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double* ms_per_step,
-                          double* bytes_per_step) {
+int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, double dd,
+                          double* ms_per_step, double* bytes_per_step) {
   if (int e = ensure_started()) return e;
   Context& c = ctx();
   if (c.op != 1) return EVSL_ERR_NOMAT;
-  if (nsteps <= 0) return EVSL_ERR_ARG;
-  const bool f32 = (prec == EVSL_PREC_MIXED);
-  if (f32)
-    if (int e = ensure_a32()) return e;
+  if (nsteps <= 0 || !(dd > 0.0) || evsl_dev::width_slot(B) < 0) return EVSL_ERR_ARG;
+  if (mode == 2 && (B & (B - 1))) return EVSL_ERR_ARG;
+  const bool f32 = (prec == EVSL_PREC_MIXED) && mode != 2;
+  FilterOp op;
+  if (int e = make_filter_op(cc, dd, f32, &op)) return e;
   const int n = c.n;
   const size_t vs = f32 ? 4 : 8;
   const size_t nb = (size_t)n * B;
@@ -255,16 +256,15 @@ int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double* ms_per_
   void* buf[5];
   for (int i = 0; i < 5; ++i) buf[i] = base + i * nb * vs;
   const evsl_dev::Mode md = (mode == 2) ? evsl_dev::kKpm : evsl_dev::kCheb;
-  const evsl_dev::Csr A = c.A.view();
   hipEvent_t e0, e1;
   EVSL_HIP(hipEventCreate(&e0));
   EVSL_HIP(hipEventCreate(&e1));
   for (int it = 0; it < nsteps + 2; ++it) {
     if (it == 2) EVSL_HIP(hipEventRecord(e0, c.stream));
-    EVSL_HIP(evsl_dev::cheb_spmm(A, B, f32, md, buf[it % 3], buf[(it + 1) % 3],
+    EVSL_HIP(evsl_dev::cheb_spmm(op.A, B, f32, md, buf[it % 3], buf[(it + 1) % 3],
                                  buf[(it + 2) % 3], buf[3], buf[4], coef.as<double>(),
-                                 coef.as<double>() + 64, 0.1, 0.5, false, red_part(),
-                                 c.stream));
+                                 coef.as<double>() + 64, op.t2, mode == 2 ? cc : op.cc, false,
+                                 red_part(), c.stream));
   }
   EVSL_HIP(hipEventRecord(e1, c.stream));
   EVSL_HIP(hipEventSynchronize(e1));
@@ -273,7 +273,7 @@ int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double* ms_per_
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *ms_per_step = ms / nsteps;
-  const int streams = (mode == 2) ? 4 : 5;
+  const int streams = (mode == 2) ? 4 : (op.cc == 0.0 ? 4 : 5);
   *bytes_per_step = (double)c.A.nnz * (vs + 4) + 4.0 * (n + 1) + (double)streams * nb * vs;
   return 0;
 }

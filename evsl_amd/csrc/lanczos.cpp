@@ -85,6 +85,8 @@ int cheb_av(const polparams* pol, const double* v, double* y, ChebWork& w, int p
   Context& c = ctx();
   const int n = c.n, m = pol->deg;
   const double cc = pol->cc, t1 = 1.0 / pol->dd, t2 = 2.0 / pol->dd;
+  (void)t1;
+  (void)t2;
   if (w.deg_loaded != m || w.mu_src != pol->mu) {
     EVSL_HIP(w.mu_d.alloc((size_t)(m + 1) * sizeof(double)));
     EVSL_HIP(hipMemcpy(w.mu_d.p, pol->mu, (size_t)(m + 1) * sizeof(double),
@@ -101,7 +103,8 @@ int cheb_av(const polparams* pol, const double* v, double* y, ChebWork& w, int p
   if (c.op == 1) {
     const bool f32 = (prec == EVSL_PREC_MIXED);
     const size_t vs = f32 ? sizeof(float) : sizeof(double);
-    if (f32) EVSL_HIP(ensure_a32() == 0 ? hipSuccess : hipErrorInvalidValue);
+    FilterOp op;
+    if (int e = make_filter_op(cc, pol->dd, f32, &op)) return e;
     DevBuf& store = f32 ? w.b32 : w.bufs;
     const int nvec = f32 ? 5 : 3;
     EVSL_HIP(store.alloc((size_t)nvec * n * vs));
@@ -115,15 +118,15 @@ int cheb_av(const polparams* pol, const double* v, double* y, ChebWork& w, int p
       EVSL_HIP(evsl_dev::f64_to_f32(n, v, (float*)v32, c.stream));
       vin = v32;
     }
-    const evsl_dev::Csr A = c.A.view();
     for (int k = 1; k <= m; ++k) {
       void* xkp1 = buf[(k - 1) % 3];
       const void* xk = (k == 1) ? vin : buf[(k - 2) % 3];
       const void* xkm1 = (k == 1) ? nullptr : (k == 2 ? vin : buf[(k - 3) % 3]);
-      EVSL_HIP(evsl_dev::cheb_spmm(A, 1, f32, evsl_dev::kCheb, xk, xkm1, xkp1, yout, nullptr,
-                                   mu_d + k, mu_d, k == 1 ? t1 : t2, cc, k == 1, nullptr,
-                                   c.stream));
-      c.astats.filter_bytes += filter_step_bytes(c, 1, (int)vs, (int)vs, k == 1 ? 3 : 5);
+      EVSL_HIP(evsl_dev::cheb_spmm(op.A, 1, f32, evsl_dev::kCheb, xk, xkm1, xkp1, yout, nullptr,
+                                   mu_d + k, mu_d, k == 1 ? op.t1 : op.t2, op.cc, k == 1,
+                                   nullptr, c.stream));
+      c.astats.filter_bytes +=
+          filter_step_bytes(c, 1, (int)vs, (int)vs, k == 1 ? 3 : (op.cc == 0.0 ? 4 : 5));
     }
     if (f32) EVSL_HIP(evsl_dev::f32_to_f64(n, (const float*)yout, y, c.stream));
     c.astats.n_filter_steps += m;

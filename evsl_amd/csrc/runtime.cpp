@@ -55,8 +55,10 @@ evsl_dev::Csr DevMatrix::view() const {
   c.ja = ja;
   c.a64 = a64;
   c.a32 = a32;
-  c.rb = rb;
-  c.nblk = nblk;
+  for (int w = 0; w < evsl_dev::kNumWidths; ++w) {
+    c.rb[w] = rb[w];
+    c.nblk[w] = nblk[w];
+  }
   return c;
 }
 
@@ -66,14 +68,16 @@ void DevMatrix::release() {
     if (ja) (void)hipFree(ja);
     if (a64) (void)hipFree(a64);
   }
-  if (rb) (void)hipFree(rb);
+  for (int w = 0; w < evsl_dev::kNumWidths; ++w)
+    if (rb[w]) (void)hipFree(rb[w]);
   if (a32) (void)hipFree(a32);
+  if (a32s) (void)hipFree(a32s);
   *this = DevMatrix();
 }
 
-std::vector<int> make_row_blocks(const int* ia, int n) {
+std::vector<int> make_row_blocks(const int* ia, int n, int rows_cap) {
   std::vector<int> rb;
-  rb.reserve(n / 64 + 2);
+  rb.reserve(n / std::max(1, rows_cap) + 2);
   int r = 0;
   while (r < n) {
     rb.push_back(r);
@@ -83,7 +87,7 @@ std::vector<int> make_row_blocks(const int* ia, int n) {
       continue;
     }
     int rows = 0, nz = 0;
-    while (r < n && rows < evsl_dev::kRowsCap) {
+    while (r < n && rows < rows_cap) {
       const int len = ia[r + 1] - ia[r];
       if (len > evsl_dev::kNnzCap || nz + len > evsl_dev::kNnzCap) break;
       nz += len;
@@ -142,9 +146,10 @@ int apply_A(const double* x, double* y) {
   return EVSL_ERR_NOMAT;
 }
 
-// register a device CSR (arrays already on the device); builds row blocks
+// register a device CSR (arrays already on the device); builds the row-block
+// tables of every block width and checks for stored diagonal entries
 static int register_device_csr(int n, int nnz, int* ia_d, int* ja_d, double* a_d,
-                               const int* ia_h, bool owns) {
+                               const int* ia_h, const int* ja_h, bool owns) {
   Context& c = g_ctx;
   c.A.release();
   c.A.n = n;
@@ -153,22 +158,93 @@ static int register_device_csr(int n, int nnz, int* ia_d, int* ja_d, double* a_d
   c.A.ja = ja_d;
   c.A.a64 = a_d;
   c.A.owns_csr = owns;
-  std::vector<int> ia_tmp;
+  std::vector<int> ia_tmp, ja_tmp;
   if (!ia_h) {
     ia_tmp.resize(n + 1);
     EVSL_HIP(hipMemcpy(ia_tmp.data(), ia_d, (n + 1) * sizeof(int), hipMemcpyDeviceToHost));
     ia_h = ia_tmp.data();
   }
-  std::vector<int> rb = make_row_blocks(ia_h, n);
-  c.A.nblk = (int)rb.size() - 1;
-  EVSL_HIP(hipMalloc((void**)&c.A.rb, rb.size() * sizeof(int)));
-  EVSL_HIP(hipMemcpy(c.A.rb, rb.data(), rb.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!ja_h) {
+    ja_tmp.resize(std::max(nnz, 1));
+    EVSL_HIP(hipMemcpy(ja_tmp.data(), ja_d, (size_t)nnz * sizeof(int), hipMemcpyDeviceToHost));
+    ja_h = ja_tmp.data();
+  }
+  bool full = true;
+  for (int r = 0; r < n && full; ++r) {
+    bool has = false;
+    for (int k = ia_h[r]; k < ia_h[r + 1]; ++k)
+      if (ja_h[k] == r) {
+        has = true;
+        break;
+      }
+    full = has;
+  }
+  c.A.full_diag = full;
+  size_t need = (size_t)evsl_dev::kRedBlocks * 64;
+  for (int w = 0; w < evsl_dev::kNumWidths; ++w) {
+    const int B = evsl_dev::width_of_slot(w);
+    std::vector<int> rb = make_row_blocks(ia_h, n, evsl_dev::rows_cap_for(B));
+    c.A.nblk[w] = (int)rb.size() - 1;
+    std::vector<int> rbp(2 * rb.size());  // (row, ia[row]) pairs
+    for (size_t i = 0; i < rb.size(); ++i) {
+      rbp[2 * i] = rb[i];
+      rbp[2 * i + 1] = ia_h[rb[i]];
+    }
+    EVSL_HIP(hipMalloc((void**)&c.A.rb[w], rbp.size() * sizeof(int)));
+    EVSL_HIP(hipMemcpy(c.A.rb[w], rbp.data(), rbp.size() * sizeof(int), hipMemcpyHostToDevice));
+    need = std::max(need, (size_t)(c.A.nblk[w] + 1) * B);
+  }
   c.n = n;
   c.op = 1;
-  // reduction scratch sized for the largest per-block partial buffers
-  size_t need = (size_t)std::max(evsl_dev::kRedBlocks * 64, (c.A.nblk + 1) * 32);
-  EVSL_HIP(c.red_part.alloc(need * sizeof(double)));
+  // reduction scratch: the largest per-row-block partial buffer, the
+  // first-level sums behind it, and one partial per 256 rows
+  need = std::max(need, (size_t)n / evsl_dev::kWG + 1);
+  EVSL_HIP(c.red_part.alloc((need + evsl_dev::kSumScratch) * sizeof(double)));
   return 0;
+}
+
+int ensure_a32_shifted(double cc, double scale, bool* ok) {
+  Context& c = g_ctx;
+  *ok = false;
+  if (c.op != 1 || !c.A.full_diag) return 0;
+  if (c.A.a32s && c.A.shift_cc == cc && c.A.shift_scale == scale) {
+    *ok = true;
+    return 0;
+  }
+  if (!c.A.a32s)
+    EVSL_HIP(hipMalloc((void**)&c.A.a32s, (size_t)std::max(c.A.nnz, 1) * sizeof(float)));
+  EVSL_HIP(evsl_dev::shift_scale_f32(c.A.view(), cc, scale, c.A.a32s, c.stream));
+  c.A.shift_cc = cc;
+  c.A.shift_scale = scale;
+  *ok = true;
+  return 0;
+}
+
+int make_filter_op(double cc, double dd, bool f32, FilterOp* op) {
+  Context& c = g_ctx;
+  op->A = c.A.view();
+  op->f32 = f32;
+  op->t1 = 1.0 / dd;
+  op->t2 = 2.0 / dd;
+  op->cc = cc;
+  if (!f32) return 0;
+  const double scale = std::ldexp(1.0, -(int)std::lround(std::log2(dd)));
+  bool ok = false;
+  if (int e = ensure_a32_shifted(cc, scale, &ok)) return e;
+  if (ok) {
+    op->A.a32 = c.A.a32s;
+    op->t1 /= scale;
+    op->t2 /= scale;
+    op->cc = 0.0;
+    return 0;
+  }
+  if (int e = ensure_a32()) return e;  // no stored diagonal: unshifted FP32 values
+  op->A.a32 = c.A.a32;
+  return 0;
+}
+
+double* red_scratch() {
+  return g_ctx.red_part.as<double>() + (g_ctx.red_part.bytes / sizeof(double) - evsl_dev::kSumScratch);
 }
 
 }  // namespace evsl
@@ -282,7 +358,7 @@ int SetAMatrix(csrMat* A) {
   EVSL_HIP(hipMemcpy(ia, A->ia, (n + 1) * sizeof(int), hipMemcpyHostToDevice));
   EVSL_HIP(hipMemcpy(ja, A->ja, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice));
   EVSL_HIP(hipMemcpy(a, A->a, (size_t)nnz * sizeof(double), hipMemcpyHostToDevice));
-  return register_device_csr(n, nnz, ia, ja, a, A->ia, true);
+  return register_device_csr(n, nnz, ia, ja, a, A->ia, A->ja, true);
 }
 
 // A->ia/ja/a are device pointers (as produced by evsl_create_csr_gpu)
@@ -291,7 +367,7 @@ int SetAMatrix_device_csr(csrMat* A) {
   if (!A || A->nrows != A->ncols || A->nrows <= 0) return EVSL_ERR_ARG;
   int nnz = 0;
   EVSL_HIP(hipMemcpy(&nnz, A->ia + A->nrows, sizeof(int), hipMemcpyDeviceToHost));
-  return register_device_csr(A->nrows, nnz, A->ia, A->ja, A->a, nullptr, false);
+  return register_device_csr(A->nrows, nnz, A->ia, A->ja, A->a, nullptr, nullptr, false);
 }
 
 void evsl_create_csr_gpu(csrMat* Acpu, csrMat* Agpu) {
@@ -329,6 +405,8 @@ int SetAMatvec(int n, MVFunc func, void* data) {
   c.A.release();
   c.n = n;
   c.op = 2;
+  EVSL_HIP(c.red_part.alloc(((size_t)evsl_dev::kRedBlocks * 64 + (size_t)n / evsl_dev::kWG + 1 +
+                             evsl_dev::kSumScratch) * sizeof(double)));
   c.func = func;
   c.func_data = data;
   return 0;
@@ -411,6 +489,11 @@ int evsl_amd_get_filter_precision(void) { return ctx().filter_prec; }
 int evsl_amd_set_refine_passes(int passes) {
   if (passes < 0 || passes > 8) return EVSL_ERR_ARG;
   ctx().refine_passes = passes;
+  return 0;
+}
+int evsl_amd_set_refine_guard(double guard) {
+  if (!(guard > 0.0 && guard <= 1.0)) return EVSL_ERR_ARG;
+  ctx().refine_guard = guard;
   return 0;
 }
 int evsl_amd_get_stats(evslAmdStats* s) {

@@ -38,8 +38,9 @@ EXPORTS = [
     "randn_double_device", "sort_double", "linspace", "evsl_timer", "StatsPrint",
     "StatsReset",
     "evsl_amd_set_filter_precision", "evsl_amd_get_filter_precision",
-    "evsl_amd_set_refine_passes", "evsl_amd_get_stats", "evsl_amd_reset_stats",
+    "evsl_amd_set_refine_passes", "evsl_amd_set_refine_guard", "evsl_amd_get_stats", "evsl_amd_reset_stats",
     "evsl_amd_last_error", "evsl_amd_stats_size", "kpmdos_ex", "evsl_amd_bench_filter",
+    "ChebLanNr_batch",
 ]
 
 
@@ -48,7 +49,7 @@ class AmdStats(C.Structure):
                 ("t_ritz", C.c_double), ("n_filter_steps", C.c_size_t),
                 ("n_filter_cols", C.c_size_t), ("n_lanczos", C.c_size_t),
                 ("n_reorth_pass", C.c_size_t), ("filter_bytes", C.c_double),
-                ("reorth_bytes", C.c_double)]
+                ("reorth_bytes", C.c_double), ("n_refine_cols", C.c_size_t)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -104,11 +105,13 @@ class EVSL:
             "evsl_amd_set_filter_precision": (I, [I]),
             "evsl_amd_get_filter_precision": (I, []),
             "evsl_amd_set_refine_passes": (I, [I]),
+            "evsl_amd_set_refine_guard": (I, [D]),
             "evsl_amd_get_stats": (I, [C.POINTER(AmdStats)]),
             "evsl_amd_reset_stats": (V, []),
             "evsl_amd_last_error": (C.c_char_p, []),
             "evsl_amd_stats_size": (C.c_size_t, []),
-            "evsl_amd_bench_filter": (I, [I, I, I, I, dp, dp]),
+            "evsl_amd_bench_filter": (I, [I, I, I, I, D, D, dp, dp]),
+            "ChebLanNr_batch": (I, [I, dp, I, D, dp, pp, ip, ddp, ddp, ddp, VP]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -245,6 +248,26 @@ class EVSL:
             return err, None, None, None
         return (0,) + self._collect(len(vinit), nev2.value, lam, W, res)
 
+    def ChebLanNr_batch(self, intvs, maxit, tol, vinit, pols, n=None, fstats=None):
+        """All slices of one matrix in one batched run.  intvs: list of
+        4-vectors, pols: list of PolParams (one per slice).  Returns
+        (err, [(lam, Y, res) per slice])."""
+        ns = len(pols)
+        iv = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64) for x in intvs]))
+        if isinstance(vinit, int):
+            vp = C.cast(vinit, c_double_p)
+            assert n is not None
+        else:
+            vinit = np.ascontiguousarray(vinit, dtype=np.float64)
+            vp = dptr(vinit)
+            n = len(vinit)
+        P = (PolParams * ns)(*pols)
+        nev = (C.c_int * ns)()
+        lam, W, res = (c_double_p * ns)(), (c_double_p * ns)(), (c_double_p * ns)()
+        err = self.lib.ChebLanNr_batch(ns, dptr(iv), maxit, tol, vp, P, nev, lam, W, res, fstats)
+        out = [self._collect(n, nev[s], lam[s], W[s], res[s]) for s in range(ns)]
+        return err, out
+
     def LanBounds(self, msteps, v):
         v = np.ascontiguousarray(v, dtype=np.float64)
         lmin, lmax = C.c_double(), C.c_double()
@@ -258,6 +281,9 @@ class EVSL:
     def set_refine_passes(self, k):
         return self.lib.evsl_amd_set_refine_passes(k)
 
+    def set_refine_guard(self, g):
+        return self.lib.evsl_amd_set_refine_guard(g)
+
     def stats(self):
         s = AmdStats()
         self.lib.evsl_amd_get_stats(C.byref(s))
@@ -266,7 +292,8 @@ class EVSL:
     def reset_stats(self):
         self.lib.evsl_amd_reset_stats()
 
-    def bench_filter(self, B, prec, mode, nsteps):
+    def bench_filter(self, B, prec, mode, nsteps, cc=6.0, dd=6.0):
         ms, by = C.c_double(), C.c_double()
-        err = self.lib.evsl_amd_bench_filter(B, prec, mode, nsteps, C.byref(ms), C.byref(by))
+        err = self.lib.evsl_amd_bench_filter(B, prec, mode, nsteps, cc, dd, C.byref(ms),
+                                             C.byref(by))
         return err, ms.value, by.value

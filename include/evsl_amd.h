@@ -31,6 +31,7 @@ typedef struct _evslAmdStats {
   size_t n_reorth_pass;  /* Gram-Schmidt passes                           */
   double filter_bytes;   /* algorithmic bytes of the filter launches      */
   double reorth_bytes;   /* algorithmic bytes of the Gram-Schmidt passes  */
+  size_t n_refine_cols;  /* Ritz vectors entering the FP64 refinement     */
 } evslAmdStats;
 
 #ifdef __cplusplus
@@ -42,6 +43,9 @@ int evsl_amd_set_filter_precision(int prec);
 int evsl_amd_get_filter_precision(void);
 /* number of FP64 filtered Rayleigh-Ritz refinement passes in MIXED mode */
 int evsl_amd_set_refine_passes(int passes);
+/* guard threshold of the refinement: Ritz values of p(A) down to
+ * guard * bar enter the FP64 Rayleigh-Ritz space (0 < guard <= 1) */
+int evsl_amd_set_refine_guard(double guard);
 
 int evsl_amd_get_stats(evslAmdStats *s);
 void evsl_amd_reset_stats(void);
@@ -54,12 +58,27 @@ size_t evsl_amd_stats_size(void);
 int kpmdos_ex(int Mdeg, int damping, int nvec, double *intv, int vec_kind,
               unsigned long long seed, double *mu, double *ecnt);
 
+/* All slices of one matrix as one batched filtered Lanczos run.  Slice s is
+ * [intvs[4s], intvs[4s+1]] with global bounds intvs[4s+2..4s+3] (identical
+ * for every slice) and filter pols[s] (from find_pol).  Each slice follows
+ * ChebLanNr (same recurrence, stopping test, acceptance); the active slices'
+ * Lanczos vectors advance together through one blocked SpMM per degree.
+ * vinit is shared by all slices (host or device).  Outputs are per slice,
+ * with ChebLanNr's ownership: lamo[s], reso[s] host (evsl_Free), Wo[s]
+ * device (evsl_Free_device).  With EVSL_PREC_MIXED the filter runs in FP32
+ * and the accepted pairs come from an FP64 filtered Rayleigh-Ritz pass. */
+int ChebLanNr_batch(int nslices, double *intvs, int maxit, double tol, double *vinit,
+                    polparams *pols, int *nevOut, double **lamo, double **Wo, double **reso,
+                    FILE *fstats);
+
 /* Measurement hook: time nsteps launches of the fused filter kernel on the
- * current matrix (block width B in {1,2,4,8,16,32}; mode 1 ChebAv step,
- * mode 2 KPM step) with HIP events.  Returns the mean launch time and the
- * algorithmic bytes one launch moves. */
-int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double *ms_per_step,
-                          double *bytes_per_step);
+ * current matrix with HIP events: block width B in {1..8,16,32}, mode 1 a
+ * ChebAv step of the filter for spectrum centre cc and half-width dd (in
+ * EVSL_PREC_MIXED on the shifted FP32 matrix), mode 2 a KPM step (FP64, B a
+ * power of two).  Returns the mean launch time and the algorithmic bytes one
+ * launch moves. */
+int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, double dd,
+                          double *ms_per_step, double *bytes_per_step);
 
 #ifdef __cplusplus
 }

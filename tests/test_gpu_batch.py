@@ -1,0 +1,87 @@
+"""GPU: the batched slice solver (ChebLanNr_batch) against the per-slice
+drop-in ChebLanNr, in FP64 (identical results) and in the mixed-precision
+form (FP32 filter, FP64 Lanczos, FP64 filtered Rayleigh-Ritz refinement).
+
+Bars: FP64 batch == plain bit for bit; mixed: identical per-slice counts,
+eigenvalues within 1e-10 relative of the FP64 plain run, residuals < tol."""
+import numpy as np
+import pytest
+
+from evsl_amd import matrices as M
+from evsl_amd.evsl import PREC_FP64, PREC_MIXED, VEC_RADEMACHER
+
+pytestmark = pytest.mark.gpu
+
+
+def setup_slices(ev, A, a, b, lo, hi, nslices, nvec=40, mdeg=300):
+    assert ev.set_matrix(A) == 0, ev.last_error()
+    intv = np.array([a, b, lo, hi])
+    err, mu, ecnt = ev.kpmdos(mdeg, 1, nvec, intv, vec_kind=VEC_RADEMACHER, seed=7)
+    assert err == 0
+    err, sli = ev.spslicer(mu, intv, nslices, int(10 * ecnt))
+    assert err == 0
+    mlan = min(max(5 * (int(1 + ecnt / nslices) + 2), 300), A.n)
+    pols, ivs = [], []
+    for s in range(nslices):
+        iv = np.array([sli[s], sli[s + 1], lo, hi])
+        pol = ev.default_pol()
+        pol.damping, pol.thresh_int, pol.thresh_ext, pol.max_deg = 2, 0.8, 0.2, 3000
+        assert ev.find_pol(iv, pol) == 0
+        pols.append(pol)
+        ivs.append(iv)
+    return ivs, pols, mlan
+
+
+CASES = {
+    "lap2d": lambda: (M.laplacian(100, 100, 1), 0.4, 0.8, 0.0, 8.0, 2),
+    "lap3d": lambda: (M.perturb_diag(M.laplacian(30, 30, 30), 1e-3, seed=5), 0.5, 0.6, None, None, 4),
+    "anderson": lambda: (M.anderson(24, 16.5, seed=4), -0.1, 0.1, None, None, 4),
+    "randsym": lambda: (M.random_symmetric(20000, 8, 32, seed=3), -0.2, 0.2, None, None, 3),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_batch_fp64_equals_plain_and_mixed_matches(ev, case):
+    A, a, b, lo, hi, ns = CASES[case]()
+    if lo is None:
+        lo, hi = M.gershgorin(A)
+    ivs, pols, mlan = setup_slices(ev, A, a, b, lo, hi, ns)
+    v0 = np.random.default_rng(11).uniform(-1, 1, A.n)
+    ev.set_precision(PREC_FP64)
+    plain = [ev.ChebLanNr(ivs[s], mlan, 1e-8, v0, pols[s]) for s in range(ns)]
+    for p in plain:
+        assert p[0] == 0, ev.last_error()
+    err, b64 = ev.ChebLanNr_batch(ivs, mlan, 1e-8, v0, pols)
+    assert err == 0, ev.last_error()
+    for (e, lp, Yp, rp), (lb, Yb, rb) in zip(plain, b64):
+        assert np.array_equal(lp, lb) and np.array_equal(rp, rb)
+    ev.set_precision(PREC_MIXED)
+    err, mx = ev.ChebLanNr_batch(ivs, mlan, 1e-8, v0, pols)
+    ev.set_precision(PREC_FP64)
+    assert err == 0, ev.last_error()
+    S = A.to_scipy()
+    for s, ((e, lp, _, _), (lm, Ym, rm)) in enumerate(zip(plain, mx)):
+        assert len(lm) == len(lp), (s, len(lm), len(lp))
+        if len(lm) == 0:
+            continue
+        np.testing.assert_allclose(np.sort(lm), np.sort(lp), rtol=1e-10, atol=1e-13)
+        assert np.all(rm < 1e-8)
+        R = S @ Ym - Ym * lm
+        np.testing.assert_allclose(np.linalg.norm(R, axis=0), rm, rtol=1e-5, atol=1e-13)
+        assert np.abs(Ym.T @ Ym - np.eye(len(lm))).max() < 1e-8
+        assert np.all((lm >= ivs[s][0] - 1e-15) & (lm <= ivs[s][1] + 1e-15))
+    for p in pols:
+        ev.free_pol(p)
+
+
+def test_batch_rejects_mixed_bounds(ev):
+    A = M.laplacian(20, 20, 1)
+    assert ev.set_matrix(A) == 0
+    p1, p2 = ev.default_pol(), ev.default_pol()
+    assert ev.find_pol(np.array([0.4, 0.8, 0.0, 8.0]), p1) == 0
+    assert ev.find_pol(np.array([0.8, 1.2, 0.0, 9.0]), p2) == 0
+    err, _ = ev.ChebLanNr_batch([np.array([0.4, 0.8, 0.0, 8.0]), np.array([0.8, 1.2, 0.0, 9.0])],
+                                300, 1e-8, np.ones(A.n), [p1, p2])
+    assert err == -104  # EVSL_ERR_ARG
+    ev.free_pol(p1)
+    ev.free_pol(p2)
