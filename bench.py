@@ -34,6 +34,22 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(kname):
+    """HBM bytes per launch of kname from the committed hardware-counter
+    summary (rocprofv3 FETCH_SIZE and WRITE_SIZE, scaled by the factors a
+    copy kernel of known size gives), or None if it was not profiled."""
+    import re
+    path = os.path.join(ROOT, "profiles", "k_cheb_spmm_pmc_lap3d100.md")
+    if not os.path.exists(path):
+        return None
+    pat = re.compile(r"^\|\s*`([^`]+)`\s*\|\s*([\d.]+)\s*\|\s*([\d.]+)\s*\|\s*([\d.]+)\s*\|")
+    for line in open(path):
+        m = pat.match(line)
+        if m and m.group(1).strip() == kname:
+            return float(m.group(4))
+    return None
+
+
 def build_problem(nx, seed):
     from evsl_amd import matrices as M
     A = M.perturb_diag(M.laplacian(nx, nx, nx), 1e-3, seed=seed)
@@ -171,7 +187,7 @@ def main():
     p.add_argument("--maxit", type=int, default=0,
                    help="cap the Krylov dimension per slice (short profiling runs only)")
     p.add_argument("--filter-only", action="store_true",
-                   help="only time the fused filter kernel and a 512 MiB device copy "
+                   help="only time the fused filter kernels and a 512 MiB device copy "
                         "(for hardware-counter runs)")
     args = p.parse_args()
     if args.gpus != 1:
@@ -188,17 +204,23 @@ def main():
     assert ev.set_matrix(A) == 0, ev.last_error()
     cc, dd = 0.5 * (lo + hi), 0.5 * (hi - lo)
     if args.filter_only:
+        # the dominant kernels at their solver widths, then a 512 MiB copy of
+        # known size that calibrates the hardware byte counters
         prec = PREC_MIXED if args.form == "mixed" else PREC_FP64
         B = args.nslices if args.form != "plain" else 1
-        err, ms, by = ev.bench_filter(B, prec, 1, 50, cc, dd)
+        out = {}
+        for tag, (b, pr, mode) in {"filter": (B, prec, 1), "refine": (32, prec, 3),
+                                   "kpm": (32, PREC_FP64, 2)}.items():
+            err, ms, by = ev.bench_filter(b, pr, mode, 50, cc, dd)
+            vt = "float" if pr == PREC_MIXED and mode != 2 else "double"
+            out[tag] = {"kernel": f"k_cheb_spmm<{vt}, {b}, {ev.spmm_vec(b, pr, mode)}, {mode}, true>",
+                        "launch_ms": ms, "bytes_per_launch": by,
+                        "achieved_GBps": by / (ms * 1e-3) / 1e9}
         nbytes = 512 << 20
-        src, dst = ev.lib._evsl_Malloc_device(nbytes), ev.lib._evsl_Malloc_device(nbytes)
-        ev.lib.evsl_memset_device(src, 0, nbytes)
-        ev.lib.evsl_memcpy_device_to_device(dst, src, nbytes)
-        ev.free_device(src)
-        ev.free_device(dst)
-        print(json.dumps({"filter_launch_ms": ms, "bytes_per_launch": by,
-                          "achieved_GBps": by / (ms * 1e-3) / 1e9, "copy_bytes": nbytes}))
+        err, msc = ev.bench_copy(nbytes, 1)
+        out["copy"] = {"kernel": "k_stream_copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                       "launch_ms": msc}
+        print(json.dumps(out))
         ev.finish()
         return
     # untimed priming: module load, first allocations
@@ -228,20 +250,25 @@ def main():
 
     # dominant kernel of the timed form: the fused Chebyshev-step SpMM
     if args.form == "mixed":
-        kB, kprec, kname = args.nslices, PREC_MIXED, f"k_cheb_spmm<float,{args.nslices},kCheb>"
+        kB, kprec, vt = args.nslices, PREC_MIXED, "float"
     elif args.form == "batch64":
-        kB, kprec, kname = args.nslices, PREC_FP64, f"k_cheb_spmm<double,{args.nslices},kCheb>"
+        kB, kprec, vt = args.nslices, PREC_FP64, "double"
     else:
-        kB, kprec, kname = 1, PREC_FP64, "k_cheb_spmm<double,1,kCheb>"
+        kB, kprec, vt = 1, PREC_FP64, "double"
+    kname = f"k_cheb_spmm<{vt}, {kB}, {ev.spmm_vec(kB, kprec, 1)}, 1, true>"
     err, ms, by = ev.bench_filter(kB, kprec, 1, 200, cc, dd)
     achieved = by / (ms * 1e-3) / 1e9
     err1, ms1, by1 = ev.bench_filter(1, PREC_FP64, 1, 200, cc, dd)
+    copy_bytes = 1 << 30
+    errc, msc = ev.bench_copy(copy_bytes, 20)
+    copy_gbps = 2 * copy_bytes / (msc * 1e-3) / 1e9
+    traffic = profiled_traffic(kname)
     st = last["stats"]
     rec = {
         "metric": METRIC, "value": value, "unit": "eigenpairs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.form != "mixed" else "f32 filter / f64 Lanczos + refinement",
+        "dtype": "f64" if args.form != "mixed" else "f32+f64",
         "data": "synthetic (seeded)",
         "config": {"workload": f"3-D Laplacian {args.nx}^3 + U[0,1e-3] diagonal, "
                                f"[{args.a},{args.b}], KPM {args.mdeg}/{args.nvec} Rademacher, "
@@ -256,8 +283,9 @@ def main():
         "breakdown_s": {"filter": st["t_filter"], "reorth": st["t_reorth"],
                         "ritz_refine": st["t_ritz"]} if args.form != "plain" else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kname, "launch_ms": ms, "bytes_per_launch": by,
+                     "copy_GBps": copy_gbps, "frac_of_copy": achieved / copy_gbps,
                      "fp64_b1_GBps": by1 / (ms1 * 1e-3) / 1e9},
     }
     if plain is not None:

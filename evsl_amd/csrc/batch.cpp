@@ -55,7 +55,8 @@ int block_filter(int B, const FilterOp& op, const double* mus_d, int maxdeg, con
     EVSL_HIP(evsl_dev::cheb_spmm(op.A, B, op.f32, evsl_dev::kCheb, xk, xkm1, xkp1, Y, nullptr,
                                  mus_d + (size_t)k * B, mus_d, k == 1 ? op.t1 : op.t2, op.cc,
                                  k == 1, nullptr, c.stream));
-    const int streams = k == 1 ? 3 : (op.cc == 0.0 ? 4 : 5);
+    // xk (gathered once), xkm1, y in; xkp1, y out (first degree: xk in, xkp1, y out)
+    const int streams = k == 1 ? 3 : 5;
     c.astats.filter_bytes += (double)c.A.nnz * (vs + 4) + 4.0 * (c.n + 1) +
                              (double)streams * B * c.n * vs;
   }
@@ -317,7 +318,7 @@ int residual_filter(int n, int m, double* U, double* W, const polparams* pol, Bl
                                      g_d.as<double>() + (size_t)k * B, mp_d.as<double>() + 2 * k,
                                      op.t2, op.cc, k == 1, nullptr, c.stream));
         c.astats.filter_bytes += (double)c.A.nnz * 8 + 4.0 * (n + 1) +
-                                 (double)(k == 1 ? 4 : 5) * B * n * 4;
+                                 (double)(k == 1 ? 4 : 6) * B * n * 4;
       }
       c.astats.n_filter_steps += deg - 1;
       c.astats.n_filter_cols += (size_t)(deg - 1) * B;
@@ -337,7 +338,7 @@ int residual_filter(int n, int m, double* U, double* W, const polparams* pol, Bl
 // Ritz block, FP64 filter pass(es), Cholesky QR and FP64 Rayleigh-Ritz.
 int refine_extract(int n, int kdim, const double* dT, const double* eT, const double* V,
                    const polparams* pol, const double* intv, double tol, bool mixed,
-                   int* nevOut, double** lamo, double** Wo, double** reso) {
+                   int* nevOut, double** lamo, double** Wo, double** reso, FILE* fstats) {
   Context& c = ctx();
   const double bar = pol->bar, aa = intv[0], bb = intv[1];
   std::vector<double> eval(kdim), e(kdim + 1, 0.0);
@@ -345,7 +346,7 @@ int refine_extract(int n, int kdim, const double* dT, const double* eT, const do
   if (kdim > 1) std::memcpy(e.data(), eT, (kdim - 1) * sizeof(double));
   RotationLog rec;
   if (int err = tridiag_ql(kdim, eval.data(), e.data(), &rec)) return err;
-  // guard threshold: keep Ritz values of p(A) down to half of bar so that
+  // guard threshold (default half of bar): keep Ritz values of p(A) so that
   // eigenvectors just outside [a, b] are in the Rayleigh-Ritz space
   const double thr = mixed ? c.refine_guard * bar : bar;
   std::vector<int> sel;
@@ -353,6 +354,12 @@ int refine_extract(int n, int kdim, const double* dT, const double* eT, const do
     if (!(eval[i] < thr)) sel.push_back(i);
   int m = (int)sel.size();
   if (mixed) c.astats.n_refine_cols += m;
+  if (fstats) {
+    int above = 0;
+    for (int i = 0; i < kdim; ++i) above += !(eval[i] < bar);
+    std::fprintf(fstats, "  extract: kdim %d, Ritz values >= bar %d, candidates %d (thr %.3f)\n",
+                 kdim, above, m, thr);
+  }
   const int alloc = std::max(kdim, m) + 1;
   double* Lam = (double*)std::malloc(alloc * sizeof(double));
   double* res = (double*)std::malloc(alloc * sizeof(double));
@@ -371,78 +378,101 @@ int refine_extract(int n, int kdim, const double* dT, const double* eT, const do
   EVSL_HIP(evsl_dev::gemm_nn(n, m, kdim, V, n, Zsel.as<double>(), kdim, Ud, n, c.stream));
   Zsel.release();
   BlockWork bw;
-  std::vector<double> lam_rr;
-  if (mixed) {
-    // filter passes on the Ritz block (FP32 residual recurrence)
-    for (int pass = 0; pass < c.refine_passes; ++pass)
-      if (int err = residual_filter(n, m, Ud, Wd, pol, bw)) return err;
-    // Cholesky QR: G = U^T U = R^T R, U <- U R^-1
-    DevBuf G, gp, M;
-    EVSL_HIP(G.alloc((size_t)m * m * sizeof(double)));
-    EVSL_HIP(gp.alloc(evsl_dev::gemm_tn_part_elems(n, m, m) * sizeof(double)));
-    EVSL_HIP(M.alloc((size_t)m * m * sizeof(double)));
-    for (int qr = 0; qr < 2; ++qr) {
-      EVSL_HIP(evsl_dev::gemm_tn(n, m, m, Ud, n, Ud, n, G.as<double>(), m, gp.as<double>(),
-                                 c.stream));
-      std::vector<double> Gh((size_t)m * m), Ri;
-      EVSL_HIP(hipMemcpy(Gh.data(), G.p, Gh.size() * sizeof(double), hipMemcpyDeviceToHost));
-      for (int j = 0; j < m; ++j)  // symmetrize
-        for (int i = 0; i < j; ++i) {
-          const double a = 0.5 * (Gh[(size_t)j * m + i] + Gh[(size_t)i * m + j]);
-          Gh[(size_t)j * m + i] = Gh[(size_t)i * m + j] = a;
-        }
-      if (!cholesky_upper(m, Gh)) {
-        set_error_msg("refinement: Ritz block lost rank in Cholesky QR");
-        return EVSL_ERR_EIG;
-      }
-      invert_upper(m, Gh, Ri);
-      EVSL_HIP(hipMemcpy(M.p, Ri.data(), Ri.size() * sizeof(double), hipMemcpyHostToDevice));
-      EVSL_HIP(evsl_dev::gemm_nn(n, m, m, Ud, n, M.as<double>(), m, Wd, n, c.stream));
-      std::swap(Ud, Wd);
-    }
-    // Rayleigh-Ritz with A in FP64: H = U^T A U
-    if (int err = apply_A_cols(n, m, Ud, Wd, bw)) return err;
-    EVSL_HIP(evsl_dev::gemm_tn(n, m, m, Ud, n, Wd, n, G.as<double>(), m, gp.as<double>(),
-                               c.stream));
-    std::vector<double> Hh((size_t)m * m), Q((size_t)m * m);
-    lam_rr.resize(m);
-    EVSL_HIP(hipMemcpy(Hh.data(), G.p, Hh.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (int j = 0; j < m; ++j)
-      for (int i = 0; i < j; ++i) {
-        const double a = 0.5 * (Hh[(size_t)j * m + i] + Hh[(size_t)i * m + j]);
-        Hh[(size_t)j * m + i] = Hh[(size_t)i * m + j] = a;
-      }
-    if (int err = sym_eig_host(m, Hh.data(), m, Q.data(), m, lam_rr.data())) return err;
-    EVSL_HIP(hipMemcpy(M.p, Q.data(), Q.size() * sizeof(double), hipMemcpyHostToDevice));
-    // rotate both U and W = A U by Q
-    DevBuf U2;
-    EVSL_HIP(U2.alloc((size_t)n * m * sizeof(double)));
-    EVSL_HIP(evsl_dev::gemm_nn(n, m, m, Ud, n, M.as<double>(), m, U2.as<double>(), n, c.stream));
-    EVSL_HIP(evsl_dev::gemm_nn(n, m, m, Wd, n, M.as<double>(), m, Ud, n, c.stream));
-    // now: U2 = Ritz vectors, Ud = A * Ritz vectors
-    EVSL_HIP(hipMemcpyAsync(Wd, U2.p, (size_t)n * m * sizeof(double), hipMemcpyDeviceToDevice,
-                            c.stream));
-    std::swap(Ud, Wd);  // Ud = Ritz vectors, Wd = A * Ritz vectors
-  }
   const int nch = (n + 8191) / 8192;
   EVSL_HIP(part.alloc((size_t)nch * m * sizeof(double)));
   EVSL_HIP(s2.alloc(m * sizeof(double)));
   EVSL_HIP(tv.alloc(m * sizeof(double)));
   EVSL_HIP(r2.alloc(m * sizeof(double)));
+  std::vector<double> hs2(m), ht(m), hr2(m);
+  // Rayleigh quotients and residual norms of the columns of Ud (A U in Wd);
+  // s2 holds the squared column norms, computed by the caller
+  auto measure = [&]() -> int {
+    EVSL_HIP(evsl_dev::col_dot(n, m, Wd, n, Ud, n, part.as<double>(), tv.as<double>(), c.stream));
+    EVSL_HIP(evsl_dev::col_res2(n, m, Wd, n, Ud, n, tv.as<double>(), part.as<double>(),
+                                r2.as<double>(), c.stream));
+    EVSL_HIP(hipMemcpy(hs2.data(), s2.p, m * sizeof(double), hipMemcpyDeviceToHost));
+    EVSL_HIP(hipMemcpy(ht.data(), tv.p, m * sizeof(double), hipMemcpyDeviceToHost));
+    EVSL_HIP(hipMemcpy(hr2.data(), r2.p, m * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+  };
   if (!mixed) {
     EVSL_HIP(evsl_dev::col_dot(n, m, Ud, n, Ud, n, part.as<double>(), s2.as<double>(), c.stream));
     EVSL_HIP(evsl_dev::col_scale_rsqrt(n, m, Ud, n, s2.as<double>(), c.stream));
     if (int err = apply_A_cols(n, m, Ud, Wd, bw)) return err;
+    if (int err = measure()) return err;
   } else {
-    EVSL_HIP(evsl_dev::col_dot(n, m, Ud, n, Ud, n, part.as<double>(), s2.as<double>(), c.stream));
+    // Refinement passes: filter the block with p(A) (FP32 residual
+    // recurrence), re-orthonormalize (Cholesky QR twice) and extract Ritz
+    // pairs with A in FP64.  Beyond the requested passes, at most
+    // kMaxExtraPasses more run while a Ritz value inside [a, b] is close to
+    // converged (tol <= residual < kRetryFactor * tol): a pass damps the
+    // components outside the guard set by at least the guard ratio.
+    constexpr int kMaxExtraPasses = 2;
+    constexpr double kRetryFactor = 1e3;
+    DevBuf G, gp, M, U2;
+    EVSL_HIP(G.alloc((size_t)m * m * sizeof(double)));
+    EVSL_HIP(gp.alloc(evsl_dev::gemm_tn_part_elems(n, m, m) * sizeof(double)));
+    EVSL_HIP(M.alloc((size_t)m * m * sizeof(double)));
+    EVSL_HIP(U2.alloc((size_t)n * m * sizeof(double)));
+    const int min_pass = std::max(c.refine_passes, 1);
+    for (int pass = 0;; ++pass) {
+      if (int err = residual_filter(n, m, Ud, Wd, pol, bw)) return err;
+      if (pass > 0) c.astats.n_refine_cols += m;
+      // Cholesky QR: G = U^T U = R^T R, U <- U R^-1
+      for (int qr = 0; qr < 2; ++qr) {
+        EVSL_HIP(evsl_dev::gemm_tn(n, m, m, Ud, n, Ud, n, G.as<double>(), m, gp.as<double>(),
+                                   c.stream));
+        std::vector<double> Gh((size_t)m * m), Ri;
+        EVSL_HIP(hipMemcpy(Gh.data(), G.p, Gh.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int j = 0; j < m; ++j)  // symmetrize
+          for (int i = 0; i < j; ++i) {
+            const double a = 0.5 * (Gh[(size_t)j * m + i] + Gh[(size_t)i * m + j]);
+            Gh[(size_t)j * m + i] = Gh[(size_t)i * m + j] = a;
+          }
+        if (!cholesky_upper(m, Gh)) {
+          set_error_msg("refinement: Ritz block lost rank in Cholesky QR");
+          return EVSL_ERR_EIG;
+        }
+        invert_upper(m, Gh, Ri);
+        EVSL_HIP(hipMemcpy(M.p, Ri.data(), Ri.size() * sizeof(double), hipMemcpyHostToDevice));
+        EVSL_HIP(evsl_dev::gemm_nn(n, m, m, Ud, n, M.as<double>(), m, Wd, n, c.stream));
+        std::swap(Ud, Wd);
+      }
+      // Rayleigh-Ritz with A in FP64: H = U^T A U
+      if (int err = apply_A_cols(n, m, Ud, Wd, bw)) return err;
+      EVSL_HIP(evsl_dev::gemm_tn(n, m, m, Ud, n, Wd, n, G.as<double>(), m, gp.as<double>(),
+                                 c.stream));
+      std::vector<double> Hh((size_t)m * m), Q((size_t)m * m), lam_rr(m);
+      EVSL_HIP(hipMemcpy(Hh.data(), G.p, Hh.size() * sizeof(double), hipMemcpyDeviceToHost));
+      for (int j = 0; j < m; ++j)
+        for (int i = 0; i < j; ++i) {
+          const double a = 0.5 * (Hh[(size_t)j * m + i] + Hh[(size_t)i * m + j]);
+          Hh[(size_t)j * m + i] = Hh[(size_t)i * m + j] = a;
+        }
+      if (int err = sym_eig_host(m, Hh.data(), m, Q.data(), m, lam_rr.data())) return err;
+      EVSL_HIP(hipMemcpy(M.p, Q.data(), Q.size() * sizeof(double), hipMemcpyHostToDevice));
+      // rotate U and W = A U by Q: U2 = Ritz vectors, Ud = A * Ritz vectors
+      EVSL_HIP(evsl_dev::gemm_nn(n, m, m, Ud, n, M.as<double>(), m, U2.as<double>(), n, c.stream));
+      EVSL_HIP(evsl_dev::gemm_nn(n, m, m, Wd, n, M.as<double>(), m, Ud, n, c.stream));
+      EVSL_HIP(hipMemcpyAsync(Wd, U2.p, (size_t)n * m * sizeof(double), hipMemcpyDeviceToDevice,
+                              c.stream));
+      std::swap(Ud, Wd);  // Ud = Ritz vectors, Wd = A * Ritz vectors
+      EVSL_HIP(evsl_dev::col_dot(n, m, Ud, n, Ud, n, part.as<double>(), s2.as<double>(),
+                                 c.stream));
+      if (int err = measure()) return err;
+      if (pass + 1 < min_pass) continue;
+      int near = 0;
+      for (int j = 0; j < m; ++j) {
+        const double r = std::sqrt(std::max(hr2[j], 0.0));
+        if (ht[j] >= aa - kEpsM && ht[j] <= bb + kEpsM && !(r < tol) && r < kRetryFactor * tol)
+          ++near;
+      }
+      if (fstats)
+        std::fprintf(fstats, "  refinement pass %d: %d near-converged in [a,b]\n", pass + 1, near);
+      if (near == 0 || pass + 1 >= min_pass + kMaxExtraPasses) break;
+      ++c.astats.n_refine_extra;
+    }
   }
-  EVSL_HIP(evsl_dev::col_dot(n, m, Wd, n, Ud, n, part.as<double>(), tv.as<double>(), c.stream));
-  EVSL_HIP(evsl_dev::col_res2(n, m, Wd, n, Ud, n, tv.as<double>(), part.as<double>(),
-                              r2.as<double>(), c.stream));
-  std::vector<double> hs2(m), ht(m), hr2(m);
-  EVSL_HIP(hipMemcpy(hs2.data(), s2.p, m * sizeof(double), hipMemcpyDeviceToHost));
-  EVSL_HIP(hipMemcpy(ht.data(), tv.p, m * sizeof(double), hipMemcpyDeviceToHost));
-  EVSL_HIP(hipMemcpy(hr2.data(), r2.p, m * sizeof(double), hipMemcpyDeviceToHost));
   std::vector<int> keep;
   int nev = 0;
   for (int j = 0; j < m; ++j) {
@@ -450,6 +480,8 @@ int refine_extract(int n, int kdim, const double* dT, const double* eT, const do
     const double t = ht[j];
     if (t < aa - kEpsM || t > bb + kEpsM) continue;
     const double r = std::sqrt(std::max(hr2[j], 0.0));
+    if (fstats && !(r < tol))
+      std::fprintf(fstats, "  rejected in [a,b]: lambda %.12f residual %.3e\n", t, r);
     if (r < tol) {
       Lam[nev] = t;
       res[nev] = r;
@@ -725,8 +757,10 @@ int ChebLanNr_batch(int nslices, double* intvs, int maxit, double tol, double* v
         L.kdim = kdim;
         L.active = false;
         const double tr = evsl_timer();
+        if (fstats) std::fprintf(fstats, "slice %d done at k %d\n", s, k);
         int err = refine_extract(n, kdim, L.dT.data(), L.eT.data(), L.V.as<double>(), L.pol,
-                                 L.intv, tol, mixed, &nevOut[s], &lamo[s], &Wo[s], &reso[s]);
+                                 L.intv, tol, mixed, &nevOut[s], &lamo[s], &Wo[s], &reso[s],
+                                 fstats);
         c.astats.t_ritz += evsl_timer() - tr;
         L.V.release();
         if (err) return err;

@@ -53,6 +53,17 @@ hipError_t cheb_spmm(const Csr& A, int B, bool fp32, Mode mode, const void* xk,
                      const double* s, const double* mu0, double t, double cc,
                      bool first, double* part, hipStream_t st);
 
+// dst = src for bytes / 16 16-byte words (the bandwidth reference kernel)
+hipError_t stream_copy(size_t bytes, const void* src, void* dst, hipStream_t st);
+
+// columns per lane of the cheb_spmm kernel used for this width, precision
+// and mode (-1 for an unsupported width)
+int cheb_spmm_vec(int B, bool fp32, Mode mode);
+
+// number of row blocks (KPM partials per column) one cheb_spmm launch of
+// this width, precision and mode uses; -1 for an unsupported width
+int cheb_spmm_nblk(const Csr& A, int B, bool fp32, Mode mode);
+
 // out[c] = sum_b part[b*B + c] over nblk row blocks, in a fixed order; uses
 // scratch (>= kSumScratch doubles) for a first level when nblk is large
 constexpr int kSumScratch = 256 * 32;

@@ -14,6 +14,8 @@
 //    partials; V h with h staged in LDS.  Both stream V once (HBM-bound).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "dev.h"
 
 namespace evsl_dev {
@@ -48,7 +50,20 @@ __device__ __forceinline__ double block_sum(double v, double* sh /*>=4*/) {
 // ---------------------------------------------------------------------
 // fused Chebyshev step SpMM
 // ---------------------------------------------------------------------
-template <typename VT, int B, int MODE>
+// V contiguous values of one row of an interleaved block (one vector load)
+template <typename VT, int V>
+struct alignas(sizeof(VT) * V) VecT {
+  VT v[V];
+};
+
+// Lane layout: L = B/V lanes share a row, each owning V adjacent columns
+// (c0 = sub*V), so a block of 256 lanes covers NG = 256/L rows and every x
+// access is one V-wide vector load.  LDS = true stages the block's ja/a
+// through LDS with coalesced loads; LDS = false reads them from global memory
+// (the L lanes of a row load the same address).  The per-column summation
+// order is the same for every (V, LDS), so results are bitwise independent
+// of the variant (the KPM block partials aside, whose tree depends on NG).
+template <typename VT, int B, int V, int MODE, bool LDS>
 __global__ __launch_bounds__(kWG) void k_cheb_spmm(
     const int* __restrict__ ia, const int* __restrict__ ja, const VT* __restrict__ av,
     const int2* __restrict__ rbp, int nblk, const VT* __restrict__ xk,
@@ -56,62 +71,92 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
     const VT* __restrict__ v0, const double* __restrict__ scol,
     const double* __restrict__ mu0col, double t_, double cc_, int first,
     double* __restrict__ part) {
-  __shared__ int s_ja[kNnzCap];
-  __shared__ VT s_a[kNnzCap];
-  __shared__ double s_red[kWG];
-  constexpr int NG = kWG / B;  // row groups; lanes >= NG*B idle when B is not a power of 2
+  static_assert(B % V == 0, "V must divide B");
+  constexpr int L = B / V;
+  constexpr int NG = kWG / L;  // lanes >= NG*L idle when L does not divide 256
+  constexpr int U = (V * sizeof(VT) >= 32) ? 4 : 8;  // gathers in flight per lane
+  using Vec = VecT<VT, V>;
+  // staging area for ja/a, reused for the group reductions once the row
+  // walk is over
+  constexpr size_t kStage = LDS ? (size_t)kNnzCap * (sizeof(int) + sizeof(VT)) : 0;
+  constexpr size_t kRed = (size_t)kWG * V * sizeof(double);
+  __shared__ __align__(16) unsigned char smem[kStage > kRed ? kStage : kRed];
+  int* s_ja = reinterpret_cast<int*>(smem);
+  VT* s_a = reinterpret_cast<VT*>(smem + (size_t)kNnzCap * sizeof(int));
+  double* s_red = reinterpret_cast<double*>(smem);
+
   const int blk = xcd_remap(blockIdx.x, nblk);
   // block table entries carry (first row, its row pointer): one load for both ends
   const int2 e0 = rbp[blk], e1 = rbp[blk + 1];
   const int r0 = e0.x, r1 = e1.x, p0 = e0.y, p1 = e1.y;
   const int nz = p1 - p0;
-  const int col = threadIdx.x % B;
-  const bool act = threadIdx.x < NG * B;
-  const int grp = threadIdx.x / B;
+  const int tid = threadIdx.x;
+  const int grp = tid / L, sub = tid % L, c0 = sub * V;
+  const bool act = tid < NG * L;
   const VT t = (VT)t_, cc = (VT)cc_;
   // kRes: v0 holds the residual block R and scol the per-column factors
   // 2*T_k(theta_c); mu0col[0..1] = (mu_k, mu_{k-1}) shared by all columns
-  VT s = 0, mu0 = 0, g = 0;
-  if (MODE == kCheb) {
-    s = (VT)scol[col];
-    if (first) mu0 = (VT)mu0col[col];
-  } else if (MODE == kRes) {
-    g = (VT)scol[col];
-    s = (VT)mu0col[0];
-    if (first) mu0 = (VT)mu0col[1];
+  VT s[V], mu0[V], g[V];
+#pragma unroll
+  for (int j = 0; j < V; ++j) {
+    s[j] = 0;
+    mu0[j] = 0;
+    g[j] = 0;
+    if (MODE == kCheb) {
+      s[j] = (VT)scol[c0 + j];
+      if (first) mu0[j] = (VT)mu0col[c0 + j];
+    } else if (MODE == kRes) {
+      g[j] = (VT)scol[c0 + j];
+      s[j] = (VT)mu0col[0];
+      if (first) mu0[j] = (VT)mu0col[1];
+    }
   }
   const bool need_xr = first || MODE == kKpm || cc_ != 0.0;
-  double dacc = 0.0;
+  double dacc[V];
+#pragma unroll
+  for (int j = 0; j < V; ++j) dacc[j] = 0.0;
 
   // epilogue operands of a row, loaded ahead of the gathers
   struct Ops {
-    VT xr, km1, yo, rr, v;
+    Vec xr, km1, yo, rr, v;
   };
+  auto ld = [](const VT* p, size_t i) { return *reinterpret_cast<const Vec*>(p + i); };
   auto prefetch = [&](int r) {
     Ops o{};
-    const size_t i = (size_t)r * B + col;
+    const size_t i = (size_t)r * B + c0;
     if (MODE == kPlain) return o;
-    o.xr = need_xr ? xk[i] : (VT)0;
-    o.km1 = xkm1 ? xkm1[i] : (VT)0;
-    if ((MODE == kCheb || MODE == kRes) && !first) o.yo = y[i];
-    if (MODE == kRes) o.rr = v0[i];
-    if (MODE == kKpm) o.v = v0[i];
+    if (need_xr) o.xr = ld(xk, i);
+    if (xkm1) o.km1 = ld(xkm1, i);
+    if ((MODE == kCheb || MODE == kRes) && !first) o.yo = ld(y, i);
+    if (MODE == kRes) o.rr = ld(v0, i);
+    if (MODE == kKpm) o.v = ld(v0, i);
     return o;
   };
-  auto epilogue = [&](int r, VT acc, const Ops& o) {
-    const size_t i = (size_t)r * B + col;
-    if (MODE == kPlain) {
-      xkp1[i] = acc;
-      return;
+  auto epilogue = [&](int r, const VT* acc, const Ops& o) {
+    const size_t i = (size_t)r * B + c0;
+    Vec w;
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+      if (MODE == kPlain) {
+        w.v[j] = acc[j];
+        continue;
+      }
+      VT wj = t * (acc[j] - cc * o.xr.v[j]) - o.km1.v[j];
+      if (MODE == kRes) wj += g[j] * o.rr.v[j];
+      w.v[j] = wj;
     }
-    VT w = t * (acc - cc * o.xr) - o.km1;
-    if (MODE == kRes) w += g * o.rr;
-    xkp1[i] = w;
+    *reinterpret_cast<Vec*>(xkp1 + i) = w;
     if (MODE == kCheb || MODE == kRes) {
-      const VT yo = first ? mu0 * o.xr : o.yo;
-      y[i] = yo + s * w;
-    } else {
-      dacc += (double)w * (double)o.v;
+      Vec yn;
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        const VT yo = first ? mu0[j] * o.xr.v[j] : o.yo.v[j];
+        yn.v[j] = yo + s[j] * w.v[j];
+      }
+      *reinterpret_cast<Vec*>(y + i) = yn;
+    } else if (MODE == kKpm) {
+#pragma unroll
+      for (int j = 0; j < V; ++j) dacc[j] += (double)w.v[j] * (double)o.v.v[j];
     }
   };
 
@@ -126,11 +171,17 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
       qf1 = ia[rf + 1] - p0;
       of = prefetch(rf);
     }
-    for (int p = threadIdx.x; p < nz; p += kWG) {
-      s_ja[p] = ja[p0 + p];
-      s_a[p] = av[p0 + p];
+    const int* jsrc = ja + p0;
+    const VT* asrc = av + p0;
+    if (LDS) {
+      for (int p = tid; p < nz; p += kWG) {
+        s_ja[p] = ja[p0 + p];
+        s_a[p] = av[p0 + p];
+      }
+      __syncthreads();
+      jsrc = s_ja;
+      asrc = s_a;
     }
-    __syncthreads();
     for (int r = rf; act && r < r1; r += NG) {
       int q0 = qf0, q1 = qf1;
       Ops o = of;
@@ -139,49 +190,82 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
         q1 = ia[r + 1] - p0;
         o = prefetch(r);
       }
-      VT acc = 0;
-      // eight independent gathers in flight per lane; zero padding leaves the
-      // sequential summation order (and its rounding) unchanged
-      for (int q = q0; q < q1; q += 8) {
-        int jj[8];
-        VT aa[8], xx[8];
+      VT acc[V];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+      for (int j = 0; j < V; ++j) acc[j] = 0;
+      // U independent gathers in flight per lane; zero padding leaves the
+      // sequential summation order (and its rounding) unchanged
+      for (int q = q0; q < q1; q += U) {
+        int jj[U];
+        VT aa[U];
+        Vec xx[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
           const bool ok = q + u < q1;
-          jj[u] = ok ? s_ja[q + u] : 0;
-          aa[u] = ok ? s_a[q + u] : (VT)0;
+          jj[u] = ok ? jsrc[q + u] : 0;
+          aa[u] = ok ? asrc[q + u] : (VT)0;
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) xx[u] = (q + u < q1) ? xk[(size_t)jj[u] * B + col] : (VT)0;
+        for (int u = 0; u < U; ++u) {
+          if (q + u < q1) {
+            xx[u] = ld(xk, (size_t)jj[u] * B + c0);
+          } else {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc += aa[u] * xx[u];
+            for (int j = 0; j < V; ++j) xx[u].v[j] = (VT)0;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int j = 0; j < V; ++j) acc[j] += aa[u] * xx[u].v[j];
       }
       epilogue(r, acc, o);
     }
+    if (MODE == kKpm && LDS) __syncthreads();  // s_red aliases the staging area
   } else {
     // one long row: all groups split its nonzeros, combine through LDS
-    VT acc = 0;
-    for (int p = p0 + grp; act && p < p1; p += NG) acc += av[p] * xk[(size_t)ja[p] * B + col];
-    s_red[threadIdx.x] = (double)acc;
+    VT acc[V];
+#pragma unroll
+    for (int j = 0; j < V; ++j) acc[j] = 0;
+    for (int p = p0 + grp; act && p < p1; p += NG) {
+      const VT a = av[p];
+      const Vec x = ld(xk, (size_t)ja[p] * B + c0);
+#pragma unroll
+      for (int j = 0; j < V; ++j) acc[j] += a * x.v[j];
+    }
+#pragma unroll
+    for (int j = 0; j < V; ++j) s_red[tid * V + j] = (double)acc[j];
     __syncthreads();
-    if (grp == 0) {
-      double tot = 0.0;
+    if (act && grp == 0) {
+      VT tot[V];
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        double sj = 0.0;
 #pragma unroll 1
-      for (int gg = 0; gg < NG; ++gg) tot += s_red[gg * B + col];
-      epilogue(r0, (VT)tot, prefetch(r0));
+        for (int gg = 0; gg < NG; ++gg) sj += s_red[(gg * L + sub) * V + j];
+        tot[j] = (VT)sj;
+      }
+      epilogue(r0, tot, prefetch(r0));
     }
     __syncthreads();
   }
 
-  if (MODE == kKpm) {  // B is a power of two here (NG*B == kWG)
-    static_assert(MODE != kKpm || (kWG % B) == 0, "KPM blocks need B | 256");
-    s_red[threadIdx.x] = dacc;
+  if (MODE == kKpm) {  // L is a power of two here (NG*L == kWG)
+    static_assert(MODE != kKpm || (kWG % L) == 0, "KPM blocks need B/V | 256");
+#pragma unroll
+    for (int j = 0; j < V; ++j) s_red[tid * V + j] = dacc[j];
     __syncthreads();
     for (int st = NG / 2; st > 0; st >>= 1) {
-      if (grp < st) s_red[threadIdx.x] += s_red[threadIdx.x + st * B];
+      if (grp < st) {
+#pragma unroll
+        for (int j = 0; j < V; ++j) s_red[tid * V + j] += s_red[(tid + st * L) * V + j];
+      }
       __syncthreads();
     }
-    if (grp == 0) part[(size_t)blk * B + col] = s_red[col];
+    if (grp == 0) {
+#pragma unroll
+      for (int j = 0; j < V; ++j) part[(size_t)blk * B + c0 + j] = s_red[tid * V + j];
+    }
   }
 }
 
@@ -793,27 +877,57 @@ inline unsigned grid_for(size_t n, int per_block, unsigned cap) {
   return (unsigned)g;
 }
 
-template <typename VT, int MODE>
-hipError_t launch_cheb_B(const Csr& A, int B, const void* xk, const void* xkm1, void* xkp1,
-                         void* y, const void* v0, const double* s, const double* mu0,
-                         double t, double cc, bool first, double* part, hipStream_t st) {
+// One fused step with an explicit lane layout (V columns per lane, LDS
+// staging on/off); the row-block table is the one built for B/V lanes per row.
+template <typename VT, int B, int V, int MODE, bool LDS>
+hipError_t launch_variant(const Csr& A, const void* xk, const void* xkm1, void* xkp1, void* y,
+                          const void* v0, const double* s, const double* mu0, double t,
+                          double cc, bool first, double* part, hipStream_t st) {
   const VT* av;
   if constexpr (sizeof(VT) == 4)
     av = A.a32;
   else
     av = A.a64;
   if (!av) return hipErrorInvalidValue;
-  const int slot = width_slot(B);
+  const int slot = width_slot(B / V);
   if (slot < 0 || !A.rb[slot]) return hipErrorInvalidValue;
   const int nblk = A.nblk[slot];
   const int2* rb = reinterpret_cast<const int2*>(A.rb[slot]);
-  dim3 grid(nblk), block(kWG);
-#define EVSL_CHEB_CASE(BB)                                                                   \
-  case BB:                                                                                   \
-    hipLaunchKernelGGL((k_cheb_spmm<VT, BB, MODE>), grid, block, 0, st, A.ia, A.ja, av,      \
-                       rb, nblk, (const VT*)xk, (const VT*)xkm1, (VT*)xkp1, (VT*)y,      \
-                       (const VT*)v0, s, mu0, t, cc, first ? 1 : 0, part);                   \
-    break;
+  hipLaunchKernelGGL((k_cheb_spmm<VT, B, V, MODE, LDS>), dim3(nblk), dim3(kWG), 0, st, A.ia,
+                     A.ja, av, rb, nblk, (const VT*)xk, (const VT*)xkm1, (VT*)xkp1, (VT*)y,
+                     (const VT*)v0, s, mu0, t, cc, first ? 1 : 0, part);
+  return hipGetLastError();
+}
+
+// Columns per lane for each (value size, block width, mode), chosen by a
+// sweep of every (V, LDS) layout on the 7-point 100^3 Laplacian on gfx950:
+// LDS staging won for every width, and blocks of 64 or more rows per 256
+// lanes did best (one column per lane gives only 256/B rows, so from B = 6
+// on, lanes take 2 or 4 adjacent columns).
+constexpr int spmm_vec(int vsize, int B, int mode) {
+  if (B == 2 || B == 6) return 2;
+  if (B == 4) return (vsize == 4 && mode == kKpm) ? 2 : 1;
+  if (B == 8) return (vsize == 8 && mode != kRes) ? 4 : 2;
+  if (B == 16) return (vsize == 4) == (mode != kKpm) ? 4 : 2;
+  if (B == 32) return (vsize == 8 && mode == kRes) ? 2 : 4;
+  return 1;  // B = 1, 3, 5, 7
+}
+
+template <typename VT, int B, int MODE>
+struct SpmmPolicy {
+  static constexpr int V = spmm_vec((int)sizeof(VT), B, MODE == kPlain ? kCheb : MODE);
+  static constexpr bool LDS = true;
+};
+
+template <typename VT, int MODE>
+hipError_t launch_cheb_B(const Csr& A, int B, const void* xk, const void* xkm1, void* xkp1,
+                         void* y, const void* v0, const double* s, const double* mu0,
+                         double t, double cc, bool first, double* part, hipStream_t st) {
+#define EVSL_CHEB_CASE(BB)                                                                 \
+  case BB:                                                                                 \
+    return launch_variant<VT, BB, SpmmPolicy<VT, BB, MODE>::V, MODE,                      \
+                          SpmmPolicy<VT, BB, MODE>::LDS>(A, xk, xkm1, xkp1, y, v0, s, mu0, t, \
+                                                         cc, first, part, st);
   if constexpr (MODE == kKpm) {
     switch (B) {
       EVSL_CHEB_CASE(1)
@@ -842,7 +956,14 @@ hipError_t launch_cheb_B(const Csr& A, int B, const void* xk, const void* xkm1, 
     }
   }
 #undef EVSL_CHEB_CASE
-  return hipGetLastError();
+}
+
+// STREAM-style copy with 16-byte accesses: the measured HBM reference the
+// filter kernel is compared with
+__global__ __launch_bounds__(kWG) void k_stream_copy(size_t n16, const uint4* __restrict__ src,
+                                                     uint4* __restrict__ dst) {
+  for (size_t i = (size_t)blockIdx.x * kWG + threadIdx.x; i < n16; i += (size_t)gridDim.x * kWG)
+    dst[i] = src[i];
 }
 
 }  // namespace
@@ -850,6 +971,55 @@ hipError_t launch_cheb_B(const Csr& A, int B, const void* xk, const void* xkm1, 
 // ---------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------
+hipError_t stream_copy(size_t bytes, const void* src, void* dst, hipStream_t st) {
+  const size_t n16 = bytes / 16;
+  if (n16 == 0) return hipSuccess;
+  const unsigned grid = grid_for(n16, kWG, 65536);
+  hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(kWG), 0, st, n16, (const uint4*)src,
+                     (uint4*)dst);
+  return hipGetLastError();
+}
+
+int cheb_spmm_vec(int B, bool fp32, Mode mode) {
+  auto vec = [&](auto vt_tag, auto mode_tag) -> int {
+    using VT = decltype(vt_tag);
+    constexpr int M = decltype(mode_tag)::value;
+    switch (B) {
+#define EVSL_VEC_CASE(BB) \
+  case BB:                \
+    return SpmmPolicy<VT, BB, M>::V;
+      EVSL_VEC_CASE(1)
+      EVSL_VEC_CASE(2)
+      EVSL_VEC_CASE(3)
+      EVSL_VEC_CASE(4)
+      EVSL_VEC_CASE(5)
+      EVSL_VEC_CASE(6)
+      EVSL_VEC_CASE(7)
+      EVSL_VEC_CASE(8)
+      EVSL_VEC_CASE(16)
+      EVSL_VEC_CASE(32)
+#undef EVSL_VEC_CASE
+      default:
+        return -1;
+    }
+  };
+  using P = std::integral_constant<int, kPlain>;
+  using C = std::integral_constant<int, kCheb>;
+  using K = std::integral_constant<int, kKpm>;
+  using R = std::integral_constant<int, kRes>;
+  if (fp32)
+    return mode == kPlain ? vec(0.0f, P{}) : mode == kCheb ? vec(0.0f, C{})
+           : mode == kKpm ? vec(0.0f, K{}) : vec(0.0f, R{});
+  return mode == kPlain ? vec(0.0, P{}) : mode == kCheb ? vec(0.0, C{})
+         : mode == kKpm ? vec(0.0, K{}) : vec(0.0, R{});
+}
+
+int cheb_spmm_nblk(const Csr& A, int B, bool fp32, Mode mode) {
+  const int V = cheb_spmm_vec(B, fp32, mode);
+  const int slot = V > 0 ? width_slot(B / V) : -1;
+  return slot < 0 ? -1 : A.nblk[slot];
+}
+
 hipError_t cheb_spmm(const Csr& A, int B, bool fp32, Mode mode, const void* xk,
                      const void* xkm1, void* xkp1, void* y, const void* v0, const double* s,
                      const double* mu0, double t, double cc, bool first, double* part,

@@ -69,6 +69,8 @@ int kpm_impl(int Mdeg, int damping, int nvec, const double* intv, int kind,
     EVSL_HIP(evsl_dev::rm_col_scale_rsqrt(n, B, x0, s2, c.stream));
     double* buf[3] = {X.as<double>(), X.as<double>() + nb, X.as<double>() + 2 * nb};
     if (c.op == 1) {
+      const int nblk_kpm = evsl_dev::cheb_spmm_nblk(A, B, false, evsl_dev::kKpm);
+      if (nblk_kpm < 0) return EVSL_ERR_ARG;
       for (int k = 0; k < Mdeg; ++k) {
         const double sc = (k == 0 ? 1.0 : 2.0) / wid;
         void* xkp1 = buf[k % 3];
@@ -76,7 +78,7 @@ int kpm_impl(int Mdeg, int damping, int nvec, const double* intv, int kind,
         const void* xkm1 = (k == 0) ? nullptr : (k == 1 ? x0 : buf[(k - 2) % 3]);
         EVSL_HIP(evsl_dev::cheb_spmm(A, B, false, evsl_dev::kKpm, xk, xkm1, xkp1, nullptr, x0,
                                      nullptr, nullptr, sc, ctr, false, red_part(), c.stream));
-        EVSL_HIP(evsl_dev::sum_block_partials(red_part(), A.nblk[evsl_dev::width_slot(B)], B,
+        EVSL_HIP(evsl_dev::sum_block_partials(red_part(), nblk_kpm, B,
                                               dots.as<double>() + (size_t)(k + 1) * B,
                                               red_scratch(), c.stream));
         c.astats.filter_bytes += (double)c.A.nnz * 12 + 4.0 * (n + 1) +
@@ -255,7 +257,8 @@ int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, doub
   EVSL_HIP(evsl_dev::fill(2 * 64, 0.01, coef.as<double>(), c.stream));
   void* buf[5];
   for (int i = 0; i < 5; ++i) buf[i] = base + i * nb * vs;
-  const evsl_dev::Mode md = (mode == 2) ? evsl_dev::kKpm : evsl_dev::kCheb;
+  const evsl_dev::Mode md =
+      mode == 2 ? evsl_dev::kKpm : (mode == 3 ? evsl_dev::kRes : evsl_dev::kCheb);
   hipEvent_t e0, e1;
   EVSL_HIP(hipEventCreate(&e0));
   EVSL_HIP(hipEventCreate(&e1));
@@ -273,8 +276,42 @@ int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, doub
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *ms_per_step = ms / nsteps;
-  const int streams = (mode == 2) ? 4 : (op.cc == 0.0 ? 4 : 5);
+  // KPM: xk (gathered once), xkm1, v0 in, xkp1 out; ChebAv: xk, xkm1, y in,
+  // xkp1, y out; residual recurrence: ChebAv's plus the residual block R
+  const int streams = mode == 2 ? 4 : (mode == 3 ? 6 : 5);
   *bytes_per_step = (double)c.A.nnz * (vs + 4) + 4.0 * (n + 1) + (double)streams * nb * vs;
+  return 0;
+}
+
+int evsl_amd_spmm_vec(int B, int prec, int mode) {
+  const evsl_dev::Mode md =
+      mode == 2 ? evsl_dev::kKpm : (mode == 3 ? evsl_dev::kRes : evsl_dev::kCheb);
+  return evsl_dev::cheb_spmm_vec(B, prec == EVSL_PREC_MIXED && mode != 2, md);
+}
+
+int evsl_amd_bench_copy(size_t bytes, int reps, double* ms_per_copy) {
+  if (int e = ensure_started()) return e;
+  if (reps <= 0 || bytes < 16 || !ms_per_copy) return EVSL_ERR_ARG;
+  Context& c = ctx();
+  bytes &= ~(size_t)15;
+  DevBuf src, dst;
+  EVSL_HIP(src.alloc(bytes));
+  EVSL_HIP(dst.alloc(bytes));
+  EVSL_HIP(evsl_dev::fill(bytes / 8, 1.0, src.as<double>(), c.stream));
+  hipEvent_t e0, e1;
+  EVSL_HIP(hipEventCreate(&e0));
+  EVSL_HIP(hipEventCreate(&e1));
+  for (int it = 0; it < reps + 2; ++it) {
+    if (it == 2) EVSL_HIP(hipEventRecord(e0, c.stream));
+    EVSL_HIP(evsl_dev::stream_copy(bytes, src.p, dst.p, c.stream));
+  }
+  EVSL_HIP(hipEventRecord(e1, c.stream));
+  EVSL_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  EVSL_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_per_copy = ms / reps;
   return 0;
 }
 

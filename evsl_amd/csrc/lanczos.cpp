@@ -126,7 +126,7 @@ int cheb_av(const polparams* pol, const double* v, double* y, ChebWork& w, int p
                                    mu_d + k, mu_d, k == 1 ? op.t1 : op.t2, op.cc, k == 1,
                                    nullptr, c.stream));
       c.astats.filter_bytes +=
-          filter_step_bytes(c, 1, (int)vs, (int)vs, k == 1 ? 3 : (op.cc == 0.0 ? 4 : 5));
+          filter_step_bytes(c, 1, (int)vs, (int)vs, k == 1 ? 3 : 5);
     }
     if (f32) EVSL_HIP(evsl_dev::f32_to_f64(n, (const float*)yout, y, c.stream));
     c.astats.n_filter_steps += m;
@@ -291,6 +291,11 @@ int ChebLanNr(double* intv, int maxit, double tol, double* vinit, polparams* pol
   if (int e = ensure_started()) return e;
   Context& c = ctx();
   if (c.op == 0) return EVSL_ERR_NOMAT;
+  // MIXED filter precision on a CSR matrix: the one-slice case of the
+  // batched solver, whose FP64 Rayleigh-Ritz refinement brings the
+  // FP32-filtered Ritz pairs to tol
+  if (c.filter_prec == EVSL_PREC_MIXED && c.op == 1)
+    return ChebLanNr_batch(1, intv, maxit, tol, vinit, pol, nevOut, lamo, Wo, reso, fstats);
   const double t_all = evsl_timer();
   const int n = c.n;
   maxit = std::min(n, maxit);
@@ -594,15 +599,517 @@ int LanDos(const int nvec, int msteps, int npts, double* xdos, double* ydos, dou
   return 0;
 }
 
-// Thick-restart filtered Lanczos: not yet on the device path.
-int ChebLanTr(int, int, double*, int, double, double*, polparams*, int* nev2, double** vals,
-              double** W, double** resW, FILE*) {
-  *nev2 = 0;
-  *vals = nullptr;
-  *W = nullptr;
-  *resW = nullptr;
-  set_error_msg("ChebLanTr: not available in this build");
-  return EVSL_ERR_UNSUPP;
+// Thick-restart filtered Lanczos (SRC/cheblanTr.c:59-707).  The basis V
+// (n x (lanm+1)), the locked block Y and the Ritz block live in HBM; the
+// arrow-head matrix T and its dense eigensolve stay on the host.  The filter
+// always runs in FP64: restarts lock Ritz pairs on their residual with A, so
+// an FP32-filtered basis would never lock pairs below its rounding level.
+int ChebLanTr(int lanm, int nev, double* intv, int maxit, double tol, double* vinit,
+              polparams* pol, int* nev2, double** vals, double** W, double** resW,
+              FILE* fstats) {
+  if (int e = ensure_started()) return e;
+  Context& c = ctx();
+  if (c.op == 0) return EVSL_ERR_NOMAT;
+  const double t_all = evsl_timer();
+  const int n = c.n;
+  const size_t nl = n;
+  lanm = std::min(lanm, n);
+  const int lanm1 = lanm + 1;
+  const size_t L1 = lanm1;
+  if (lanm == n) maxit = std::min(maxit, n);
+  const double nevInc = 0.2;
+  const int cycle = 50, min_inner_step = 5;
+  if (check_intv(intv) < 0) {
+    *nev2 = 0;
+    *vals = nullptr;
+    *W = nullptr;
+    *resW = nullptr;
+    return 0;
+  }
+  const double aa = intv[0], bb = intv[1];
+  const double gamB = pol->gam, bar = pol->bar;
+  if (gamB > 1.0 || gamB < -1.0) {
+    if (fstats) std::fprintf(fstats, "gamB error %.15e\n", gamB);
+    return -1;
+  }
+  if (lanm < 1) {
+    set_error_msg("ChebLanTr: lanm must be positive");
+    return EVSL_ERR_ARG;
+  }
+  if (fstats) std::fprintf(fstats, " Cheb-LanTR, dim %d  cycle %d  \n", lanm, cycle);
+  const int prec = EVSL_PREC_FP64;
+  int ycap = std::max(nev, 1);
+  LanWork L;
+  L.nch = evsl_dev::gemv_t_chunks(n);
+  int gcap = 0;  // columns the Gram-Schmidt scratch holds
+  auto ensure_gs = [&](int cols) -> int {
+    if (cols <= gcap) return 0;
+    gcap = cols;
+    EVSL_HIP(L.gt_part.alloc((size_t)L.nch * (gcap + 1) * sizeof(double)));
+    EVSL_HIP(L.h.alloc((size_t)(gcap + 1) * sizeof(double)));
+    return 0;
+  };
+  if (int e = ensure_gs(std::max(lanm1, ycap))) return e;
+  EVSL_HIP(L.V.alloc(nl * L1 * sizeof(double)));
+  EVSL_HIP(L.vrand.alloc(nl * sizeof(double)));
+  DevBuf Yd, Rvec, Wblk, Ed, part, s2d, tvd, r2d;
+  EVSL_HIP(Yd.alloc(nl * ycap * sizeof(double)));
+  EVSL_HIP(Rvec.alloc(nl * lanm * sizeof(double)));
+  constexpr int kChunk = 64;  // Ritz columns per A-application block
+  EVSL_HIP(Wblk.alloc(nl * std::min(kChunk, lanm) * sizeof(double)));
+  const int nch8 = (n + 8191) / 8192;
+  EVSL_HIP(part.alloc((size_t)nch8 * lanm * sizeof(double)));
+  EVSL_HIP(s2d.alloc(lanm * sizeof(double)));
+  EVSL_HIP(tvd.alloc(lanm * sizeof(double)));
+  EVSL_HIP(r2d.alloc(lanm * sizeof(double)));
+  double* V = L.V.as<double>();
+  std::vector<double> T(L1 * L1, 0.0), EvecT(L1 * L1), Rval(lanm), resi(lanm), s(lanm);
+  std::vector<double> Lam(ycap), res(ycap);
+  DevBuf vtmp;
+  const double* v0;
+  if (int e = as_device_vector(vinit, n, vtmp, &v0)) return e;
+  EventClock clk;
+  int lock = 0, trlen = 0, prtrlen = -1, it = 0;
+  // filter the start vector and normalise it (FILTER_VINIT)
+  if (int e = cheb_av(pol, v0, V, L.cw, prec)) return e;
+  double nv;
+  if (int e = nrm2_dev(n, V, &nv)) return e;
+  EVSL_HIP(evsl_dev::scal(n, 1.0 / nv, V, c.stream));
+  // deflation against the locked block: v -= Y Y^T v (two CGS passes at most)
+  auto deflate = [&](double* vnew) -> int {
+    if (lock == 0) return 0;
+    double nn, dummy;
+    if (int e = nrm2_dev(n, vnew, &nn)) return e;
+    return cgs_dgks(n, lock, Yd.as<double>(), vnew, nn * nn, &dummy, L);
+  };
+  // lucky breakdown: a filtered random vector orthogonalised against Y and V(:,0:k)
+  auto breakdown = [&](int k, double* vnew, double* beta) -> int {
+    rand_double_device(n, L.vrand.as<double>());
+    if (int e = cheb_av(pol, L.vrand.as<double>(), vnew, L.cw, prec)) return e;
+    if (int e = deflate(vnew)) return e;
+    double nn;
+    if (int e = nrm2_dev(n, vnew, &nn)) return e;
+    if (int e = cgs_dgks(n, k, V, vnew, nn * nn, beta, L)) return e;
+    EVSL_HIP(evsl_dev::scal(n, 1.0 / *beta, vnew, c.stream));
+    *beta = 0.0;
+    return 0;
+  };
+  while (it < maxit) {
+    double wn = 0.0, beta = 0.0;
+    int nwn = 0;
+    int k = trlen > 0 ? trlen + 1 : 0;
+    if (k + min_inner_step > lanm1) {  // the reference exits here
+      if (fstats) std::fprintf(fstats, "Krylov dim too small for this problem. Try a larger dim\n");
+      set_error_msg("ChebLanTr: Krylov dimension too small for the restart set");
+      return EVSL_ERR_ARG;
+    }
+    if (trlen > 0) {  // thick-restart special step (SRC/cheblanTr.c:224-319)
+      const int k1 = k - 1;
+      double* v = V + (size_t)k1 * nl;
+      double* vnew = v + nl;
+      clk.mark(0);
+      if (int e = cheb_av(pol, v, vnew, L.cw, prec)) return e;
+      clk.mark(1);
+      c.astats.t_filter += clk.secs(0, 1);
+      if (int e = deflate(vnew)) return e;
+      for (int i = 0; i < trlen; ++i) {
+        T[i * L1 + i] = Rval[i];
+        wn += std::fabs(Rval[i]);
+      }
+      EVSL_HIP(evsl_dev::dot_partial(n, v, vnew, red_part(), c.stream));
+      EVSL_HIP(evsl_dev::sum_scalar(red_part(), evsl_dev::kRedBlocks, red_scalar(0), c.stream));
+      if (int e = read_scalars(red_scalar(0), 1, &s[k1])) return e;
+      // vnew -= V(:, 0:k) s(0:k), with ||vnew||^2
+      EVSL_HIP(hipMemcpy(L.h.p, s.data(), (size_t)k * sizeof(double), hipMemcpyHostToDevice));
+      const int nb = (n + evsl_dev::kWG - 1) / evsl_dev::kWG;
+      EVSL_HIP(evsl_dev::gemv_n_sub_nrm(n, k, V, n, L.h.as<double>(), vnew, red_part(), c.stream));
+      EVSL_HIP(evsl_dev::sum_scalar(red_part(), nb, red_scalar(1), c.stream));
+      for (int i = 0; i < k1; ++i) {
+        T[trlen * L1 + i] = s[i];
+        T[i * L1 + trlen] = s[i];
+        wn += 2.0 * std::fabs(s[i]);
+      }
+      T[trlen * L1 + trlen] = s[k1];
+      wn += std::fabs(s[k1]);
+      double b2;
+      if (int e = read_scalars(red_scalar(1), 1, &b2)) return e;
+      beta = std::sqrt(b2);
+      wn += 2.0 * beta;
+      nwn += 3 * k;
+      if (beta * nwn < kOrthTol * wn) {
+        if (fstats) std::fprintf(fstats, "it %4d: Lucky breakdown, beta = %.15e\n", k, beta);
+        if (int e = breakdown(k, vnew, &beta)) return e;
+      } else {
+        EVSL_HIP(evsl_dev::scal(n, 1.0 / beta, vnew, c.stream));
+      }
+      T[k1 * L1 + k] = beta;
+      T[k * L1 + k1] = beta;
+    }
+    const int Ntest = std::max(20, nev - lock + 10);
+    int last_jl = 0;
+    double last_tr = 0.0;
+    const double* zold = k > 0 ? V + (size_t)(k - 1) * nl : nullptr;
+    while (k < lanm && it < maxit) {  // Lanczos inner loop (SRC/cheblanTr.c:331-503)
+      ++k;
+      double* v = V + (size_t)(k - 1) * nl;
+      double* vnew = v + nl;
+      clk.mark(0);
+      if (int e = cheb_av(pol, v, vnew, L.cw, prec)) return e;
+      clk.mark(1);
+      ++it;
+      if (int e = deflate(vnew)) return e;
+      // vnew -= beta*zold; alpha = v.vnew; vnew -= alpha*v; ||vnew||^2
+      if (zold)
+        EVSL_HIP(evsl_dev::axpy_dot(n, -beta, zold, vnew, v, red_part(), c.stream));
+      else
+        EVSL_HIP(evsl_dev::dot_partial(n, vnew, v, red_part(), c.stream));
+      EVSL_HIP(evsl_dev::sum_scalar(red_part(), evsl_dev::kRedBlocks, red_scalar(0), c.stream));
+      EVSL_HIP(evsl_dev::axpy_dev_nrm(n, -1.0, red_scalar(0), v, vnew, red_part(), c.stream));
+      EVSL_HIP(evsl_dev::sum_scalar(red_part(), evsl_dev::kRedBlocks, red_scalar(1), c.stream));
+      double sc[2];
+      if (int e = read_scalars(red_scalar(0), 2, sc)) return e;
+      const double alpha = sc[0];
+      T[(k - 1) * L1 + (k - 1)] = alpha;
+      wn += std::fabs(alpha);
+      if (int e = cgs_dgks(n, k, V, vnew, sc[1], &beta, L)) return e;
+      clk.mark(2);
+      wn += 2.0 * beta;
+      nwn += 3;
+      zold = v;
+      if (beta * nwn < kOrthTol * wn) {
+        if (fstats) std::fprintf(fstats, "it %4d: Lucky breakdown, beta = %.15e\n", it, beta);
+        if (int e = breakdown(k, vnew, &beta)) return e;
+      } else {
+        EVSL_HIP(evsl_dev::scal(n, 1.0 / beta, vnew, c.stream));
+      }
+      T[k * L1 + (k - 1)] = beta;
+      T[(k - 1) * L1 + k] = beta;
+      c.astats.t_filter += clk.secs(0, 1);
+      c.astats.t_reorth += clk.secs(1, 2);
+      c.astats.n_lanczos++;
+      const int k1 = k - trlen - Ntest;
+      if (((k1 >= 0) && (k1 % cycle == 0)) || (k == lanm) || it == maxit) {
+        const double te = evsl_timer();
+        if (int e = sym_eig_host(k, T.data(), lanm1, EvecT.data(), lanm1, Rval.data())) return e;
+        c.astats.t_eig += evsl_timer() - te;
+        if (k == lanm || it == maxit) break;
+        int jl = 0;
+        double tr = 0.0;
+        for (int i = 0; i < k; ++i)
+          if (Rval[i] + kEpsM >= bar) {
+            ++jl;
+            tr += Rval[i];
+          }
+        if (std::fabs(tr - last_tr) <= 2e-12 && jl == last_jl) {
+          if (fstats)
+            std::fprintf(fstats,
+                         "break: [it %d, k %d]: last_tr %.15e, tr %.15e, last_jl %d  jl %d\n", it,
+                         k, last_tr, tr, last_jl, jl);
+          break;
+        }
+        if (fstats)
+          std::fprintf(fstats, "[it %d, k %d] testing: last_tr %.15e, tr %.15e, last_jl %d  jl %d\n",
+                       it, k, last_tr, tr, last_jl, jl);
+        last_tr = tr;
+        last_jl = jl;
+      }
+    }
+    const double tr0 = evsl_timer();
+    // pass 1: Ritz values of p(A) at or above bar, moved to the front
+    int jl = 0;
+    for (int i = 0; i < k; ++i) {
+      if (Rval[i] + kEpsM >= bar) {
+        if (i != jl) {
+          std::memcpy(&EvecT[jl * L1], &EvecT[i * L1], (size_t)k * sizeof(double));
+          Rval[jl] = Rval[i];
+        }
+        resi[jl] = std::fabs(beta * EvecT[i * L1 + (k - 1)]);
+        ++jl;
+      }
+    }
+    // Rvec(:, 0:jl) = V(:, 0:k) EvecT(0:k, 0:jl)
+    double* R = Rvec.as<double>();
+    std::vector<double> hs2(jl), ht(jl), hr2(jl);
+    if (jl > 0) {
+      std::vector<double> Eh((size_t)k * jl);
+      for (int j = 0; j < jl; ++j)
+        std::memcpy(&Eh[(size_t)j * k], &EvecT[j * L1], (size_t)k * sizeof(double));
+      EVSL_HIP(Ed.alloc(Eh.size() * sizeof(double)));
+      EVSL_HIP(hipMemcpy(Ed.p, Eh.data(), Eh.size() * sizeof(double), hipMemcpyHostToDevice));
+      EVSL_HIP(evsl_dev::gemm_nn(n, jl, k, V, n, Ed.as<double>(), k, R, n, c.stream));
+      EVSL_HIP(evsl_dev::col_dot(n, jl, R, n, R, n, part.as<double>(), s2d.as<double>(), c.stream));
+      EVSL_HIP(hipMemcpy(hs2.data(), s2d.p, jl * sizeof(double), hipMemcpyDeviceToHost));
+      for (int i = 0; i < jl; ++i)
+        if (hs2[i] == 0.0) return 2;  // zero Ritz vector (SRC/cheblanTr.c:562-564)
+      EVSL_HIP(evsl_dev::col_scale_rsqrt(n, jl, R, n, s2d.as<double>(), c.stream));
+      // Rayleigh quotients and residuals, kChunk columns at a time
+      double* Wb = Wblk.as<double>();
+      for (int j0 = 0; j0 < jl; j0 += kChunk) {
+        const int m = std::min(kChunk, jl - j0);
+        for (int j = 0; j < m; ++j)
+          if (int e = apply_A(R + (size_t)(j0 + j) * nl, Wb + (size_t)j * nl)) return e;
+        EVSL_HIP(evsl_dev::col_dot(n, m, Wb, n, R + (size_t)j0 * nl, n, part.as<double>(),
+                                   tvd.as<double>() + j0, c.stream));
+        EVSL_HIP(evsl_dev::col_res2(n, m, Wb, n, R + (size_t)j0 * nl, n, tvd.as<double>() + j0,
+                                    part.as<double>(), r2d.as<double>() + j0, c.stream));
+      }
+      EVSL_HIP(hipMemcpy(ht.data(), tvd.p, jl * sizeof(double), hipMemcpyDeviceToHost));
+      EVSL_HIP(hipMemcpy(hr2.data(), r2d.p, jl * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    // pass 2: Ritz values of A in [a, b] are locked or kept for the restart
+    int ll = 0;
+    prtrlen = trlen;
+    trlen = 0;
+    for (int i = 0; i < jl; ++i) {
+      const double t3 = ht[i];
+      if (!(t3 >= aa - kEpsM && t3 <= bb + kEpsM)) continue;
+      ++ll;
+      const double res0 = std::sqrt(std::max(hr2[i], 0.0));
+      const double* y = R + (size_t)i * nl;
+      if (res0 < tol) {
+        if (lock >= nev) {  // grow the locked block by 1 + 20%
+          nev += 1 + (int)(nev * nevInc);
+          if (fstats) std::fprintf(fstats, "-- More eigval found: realloc space for %d evs\n", nev);
+        }
+        if (lock >= ycap) {
+          const int ncap = std::max(nev, lock + 1);
+          DevBuf Y2;
+          EVSL_HIP(Y2.alloc(nl * ncap * sizeof(double)));
+          EVSL_HIP(hipMemcpyAsync(Y2.p, Yd.p, nl * lock * sizeof(double),
+                                  hipMemcpyDeviceToDevice, c.stream));
+          std::swap(Yd.p, Y2.p);
+          std::swap(Yd.bytes, Y2.bytes);
+          ycap = ncap;
+          Lam.resize(ycap);
+          res.resize(ycap);
+          if (int e = ensure_gs(std::max(lanm1, ycap))) return e;
+        }
+        EVSL_HIP(hipMemcpyAsync(Yd.as<double>() + (size_t)lock * nl, y, nl * sizeof(double),
+                                hipMemcpyDeviceToDevice, c.stream));
+        Lam[lock] = t3;
+        res[lock] = res0;
+        ++lock;
+      } else {
+        Rval[trlen] = Rval[i];
+        EVSL_HIP(hipMemcpyAsync(V + (size_t)trlen * nl, y, nl * sizeof(double),
+                                hipMemcpyDeviceToDevice, c.stream));
+        s[trlen] = beta * EvecT[i * L1 + (k - 1)];
+        ++trlen;
+      }
+    }
+    c.astats.t_ritz += evsl_timer() - tr0;
+    if (fstats)
+      std::fprintf(fstats, "it %4d:  k %3d, jl %3d, ll %3d, lock %3d, trlen %3d\n", it, k, jl, ll,
+                   lock, trlen);
+    if (prtrlen == 0 && ll == 0) {
+      if (fstats)
+        std::fprintf(fstats, "--------------------------------------------------\n"
+                             " --> No ev.s left to be computed\n");
+      break;
+    }
+    if (it == maxit) {
+      if (fstats)
+        std::fprintf(fstats, "--------------------------------------------------\n"
+                             " --> Max its reached without convergence\n");
+      break;
+    }
+    std::fill(T.begin(), T.end(), 0.0);
+    if (trlen != k)
+      EVSL_HIP(hipMemcpyAsync(V + (size_t)trlen * nl, V + (size_t)k * nl, nl * sizeof(double),
+                              hipMemcpyDeviceToDevice, c.stream));
+  }
+  if (fstats)
+    std::fprintf(fstats, "     Number of evals found = %d\n"
+                         "--------------------------------------------------\n", lock);
+  EVSL_HIP(hipStreamSynchronize(c.stream));
+  // outputs: host Lam/res (evsl_Malloc), device Y (evsl_Malloc_device)
+  double* lam_o = (double*)std::malloc((size_t)std::max(lock, 1) * sizeof(double));
+  double* res_o = (double*)std::malloc((size_t)std::max(lock, 1) * sizeof(double));
+  double* Y_o = (double*)_evsl_Malloc_device(nl * std::max(lock, 1) * sizeof(double));
+  if (!lam_o || !res_o || !Y_o) {
+    std::free(lam_o);
+    std::free(res_o);
+    if (Y_o) _evsl_Free_device(Y_o);
+    return EVSL_ERR_ALLOC;
+  }
+  if (lock > 0) {
+    std::memcpy(lam_o, Lam.data(), lock * sizeof(double));
+    std::memcpy(res_o, res.data(), lock * sizeof(double));
+    EVSL_HIP(hipMemcpy(Y_o, Yd.p, nl * lock * sizeof(double), hipMemcpyDeviceToDevice));
+  }
+  *nev2 = lock;
+  *vals = lam_o;
+  *W = Y_o;
+  *resW = res_o;
+  evslstat.t_iter = evsl_timer() - t_all;
+  return 0;
+}
+
+
+// Thick-restart Lanczos spectrum bounds (SRC/lanTrbounds.c:35-421): Lanczos
+// on A with full reorthogonalization, restarted from the two extreme Ritz
+// pairs until the bound corrections fall below tol * (|lmin| + |lmax|).
+// bndtype <= 1: simple bounds |beta s|; otherwise Kato-Temple.
+int LanTrbounds(int lanm, int maxit, double tol, double* vinit, int bndtype, double* lammin,
+                double* lammax, FILE* fstats) {
+  if (int e = ensure_started()) return e;
+  Context& c = ctx();
+  if (c.op == 0) return EVSL_ERR_NOMAT;
+  const int n = c.n;
+  const size_t nl = n;
+  lanm = std::min(lanm, n);
+  const int lanm1 = lanm + 1;
+  const size_t L1 = lanm1;
+  if (lanm == n) maxit = std::min(maxit, n);
+  const int min_inner_step = 5;
+  if (fstats) std::fprintf(fstats, " LanTR for bounds: dim %d, maxits %d\n", lanm, maxit);
+  LanWork L;
+  L.nch = evsl_dev::gemv_t_chunks(n);
+  EVSL_HIP(L.gt_part.alloc((size_t)L.nch * (lanm1 + 1) * sizeof(double)));
+  EVSL_HIP(L.h.alloc((size_t)(lanm1 + 1) * sizeof(double)));
+  EVSL_HIP(L.V.alloc(nl * L1 * sizeof(double)));
+  DevBuf Rvec, Ed;
+  EVSL_HIP(Rvec.alloc(2 * nl * sizeof(double)));
+  double* V = L.V.as<double>();
+  std::vector<double> T(L1 * L1, 0.0), EvecT(L1 * L1), Rval(std::max(lanm, 2));
+  double s[3] = {0.0, 0.0, 0.0};
+  DevBuf vtmp;
+  const double* v0;
+  if (int e = as_device_vector(vinit, n, vtmp, &v0)) return e;
+  EVSL_HIP(hipMemcpyAsync(V, v0, nl * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  double nv;
+  if (int e = nrm2_dev(n, V, &nv)) return e;
+  EVSL_HIP(evsl_dev::scal(n, 1.0 / nv, V, c.stream));
+  const int nb = (n + evsl_dev::kWG - 1) / evsl_dev::kWG;
+  // lucky breakdown: a random vector orthogonalised against V(:, 0:k)
+  auto breakdown = [&](int k, double* vnew, double* beta) -> int {
+    rand_double_device(n, vnew);
+    double nn;
+    if (int e = nrm2_dev(n, vnew, &nn)) return e;
+    if (int e = cgs_dgks(n, k, V, vnew, nn * nn, beta, L)) return e;
+    EVSL_HIP(evsl_dev::scal(n, 1.0 / *beta, vnew, c.stream));
+    *beta = 0.0;
+    return 0;
+  };
+  double lmin = 0.0, lmax = 0.0;
+  int it = 0, trlen = 0;
+  while (it < maxit) {
+    double wn = 0.0, beta = 0.0;
+    int nwn = 0;
+    int k = trlen > 0 ? trlen + 1 : 0;
+    if (k + min_inner_step > lanm1) {  // the reference exits here
+      set_error_msg("LanTrbounds: Krylov dimension too small");
+      return EVSL_ERR_ARG;
+    }
+    if (trlen > 0) {  // thick-restart special step (SRC/lanTrbounds.c:145-215)
+      const int k1 = k - 1;
+      double* v = V + (size_t)k1 * nl;
+      double* vnew = v + nl;
+      if (int e = apply_A(v, vnew)) return e;
+      for (int i = 0; i < trlen; ++i) {
+        T[i * L1 + i] = Rval[i];
+        wn += std::fabs(Rval[i]);
+      }
+      EVSL_HIP(evsl_dev::dot_partial(n, v, vnew, red_part(), c.stream));
+      EVSL_HIP(evsl_dev::sum_scalar(red_part(), evsl_dev::kRedBlocks, red_scalar(0), c.stream));
+      if (int e = read_scalars(red_scalar(0), 1, &s[k1])) return e;
+      EVSL_HIP(hipMemcpy(L.h.p, s, (size_t)k * sizeof(double), hipMemcpyHostToDevice));
+      EVSL_HIP(evsl_dev::gemv_n_sub_nrm(n, k, V, n, L.h.as<double>(), vnew, red_part(), c.stream));
+      EVSL_HIP(evsl_dev::sum_scalar(red_part(), nb, red_scalar(1), c.stream));
+      for (int i = 0; i < k1; ++i) {
+        T[trlen * L1 + i] = s[i];
+        T[i * L1 + trlen] = s[i];
+        wn += 2.0 * std::fabs(s[i]);
+      }
+      T[trlen * L1 + trlen] = s[k1];
+      wn += std::fabs(s[k1]);
+      double b2;
+      if (int e = read_scalars(red_scalar(1), 1, &b2)) return e;
+      beta = std::sqrt(b2);
+      wn += 2.0 * beta;
+      nwn += 3 * k;
+      if (beta * nwn < kOrthTol * wn) {
+        if (int e = breakdown(k, vnew, &beta)) return e;
+      } else {
+        EVSL_HIP(evsl_dev::scal(n, 1.0 / beta, vnew, c.stream));
+      }
+      T[k1 * L1 + k] = beta;
+      T[k * L1 + k1] = beta;
+    }
+    const double* zold = k > 0 ? V + (size_t)(k - 1) * nl : nullptr;
+    while (k < lanm && it < maxit) {  // Lanczos inner loop (SRC/lanTrbounds.c:222-300)
+      ++k;
+      double* v = V + (size_t)(k - 1) * nl;
+      double* vnew = v + nl;
+      if (int e = apply_A(v, vnew)) return e;
+      ++it;
+      if (zold)
+        EVSL_HIP(evsl_dev::axpy_dot(n, -beta, zold, vnew, v, red_part(), c.stream));
+      else
+        EVSL_HIP(evsl_dev::dot_partial(n, vnew, v, red_part(), c.stream));
+      EVSL_HIP(evsl_dev::sum_scalar(red_part(), evsl_dev::kRedBlocks, red_scalar(0), c.stream));
+      EVSL_HIP(evsl_dev::axpy_dev_nrm(n, -1.0, red_scalar(0), v, vnew, red_part(), c.stream));
+      EVSL_HIP(evsl_dev::sum_scalar(red_part(), evsl_dev::kRedBlocks, red_scalar(1), c.stream));
+      double sc[2];
+      if (int e = read_scalars(red_scalar(0), 2, sc)) return e;
+      T[(k - 1) * L1 + (k - 1)] = sc[0];
+      wn += std::fabs(sc[0]);
+      if (int e = cgs_dgks(n, k, V, vnew, sc[1], &beta, L)) return e;
+      wn += 2.0 * beta;
+      nwn += 3;
+      zold = v;
+      if (beta * nwn < kOrthTol * wn) {
+        if (fstats) std::fprintf(fstats, "it %4d: Lucky breakdown, beta = %.15e\n", it, beta);
+        if (int e = breakdown(k, vnew, &beta)) return e;
+      } else {
+        EVSL_HIP(evsl_dev::scal(n, 1.0 / beta, vnew, c.stream));
+      }
+      T[k * L1 + (k - 1)] = beta;
+      T[(k - 1) * L1 + k] = beta;
+      c.astats.n_lanczos++;
+    }
+    if (k < 2) {
+      set_error_msg("LanTrbounds: fewer than two Lanczos steps");
+      return EVSL_ERR_ARG;
+    }
+    if (int e = sym_eig_host(k, T.data(), lanm1, EvecT.data(), lanm1, Rval.data())) return e;
+    // spikes: beta times the last components of the extreme eigenvectors
+    s[0] = beta * EvecT[k - 1];
+    s[1] = beta * EvecT[(size_t)(k - 1) * L1 + (k - 1)];
+    double t1, t2;
+    if (bndtype <= 1) {
+      t1 = std::fabs(s[0]);
+      t2 = std::fabs(s[1]);
+    } else {  // Kato-Temple
+      t1 = 2.0 * s[0] * s[0] / (Rval[1] - Rval[0]);
+      t2 = 2.0 * s[1] * s[1] / (Rval[k - 1] - Rval[k - 2]);
+    }
+    lmin = Rval[0] - t1;
+    lmax = Rval[k - 1] + t2;
+    // the two extreme Ritz vectors become the restart set
+    std::vector<double> Eh((size_t)2 * k);
+    std::memcpy(Eh.data(), &EvecT[0], (size_t)k * sizeof(double));
+    std::memcpy(Eh.data() + k, &EvecT[(size_t)(k - 1) * L1], (size_t)k * sizeof(double));
+    EVSL_HIP(Ed.alloc(Eh.size() * sizeof(double)));
+    EVSL_HIP(hipMemcpy(Ed.p, Eh.data(), Eh.size() * sizeof(double), hipMemcpyHostToDevice));
+    EVSL_HIP(evsl_dev::gemm_nn(n, 2, k, V, n, Ed.as<double>(), k, Rvec.as<double>(), n,
+                               c.stream));
+    trlen = 2;
+    EVSL_HIP(hipMemcpyAsync(V, Rvec.p, 2 * nl * sizeof(double), hipMemcpyDeviceToDevice,
+                            c.stream));
+    Rval[1] = Rval[k - 1];
+    if (fstats)
+      std::fprintf(fstats, "it %4d, k %3d: ritz %.15e %.15e, t1,t2 %e %e, res %.15e %.15e\n", it,
+                   k, Rval[0], Rval[1], t1, t2, std::fabs(s[0]), std::fabs(s[1]));
+    if (t1 + t2 < tol * (std::fabs(lmin) + std::fabs(lmax))) break;
+    std::fill(T.begin(), T.end(), 0.0);
+    EVSL_HIP(hipMemcpyAsync(V + (size_t)trlen * nl, V + (size_t)k * nl, nl * sizeof(double),
+                            hipMemcpyDeviceToDevice, c.stream));
+  }
+  EVSL_HIP(hipStreamSynchronize(c.stream));
+  *lammin = lmin;
+  *lammax = lmax;
+  return 0;
 }
 
 }  // extern "C"

@@ -487,7 +487,7 @@ int evsl_amd_set_filter_precision(int prec) {
 }
 int evsl_amd_get_filter_precision(void) { return ctx().filter_prec; }
 int evsl_amd_set_refine_passes(int passes) {
-  if (passes < 0 || passes > 8) return EVSL_ERR_ARG;
+  if (passes < 1 || passes > 8) return EVSL_ERR_ARG;
   ctx().refine_passes = passes;
   return 0;
 }

@@ -1,7 +1,10 @@
 // sparse_host.cpp -- host-side sparse/vector helpers of the C interface:
 // COO -> CSR with column-sorted rows, CSR copies, host CSR matvec, sorting
-// and linspace (SRC/spmat.c:27-322, SRC/vect.c:82-141 of EVSL 1.1.1).
+// and linspace (SRC/spmat.c:27-322, SRC/vect.c:82-141 of EVSL 1.1.1), and the
+// drivers' Matrix Market reader (TESTS/COMMON/io.c:115-217).
 #include <algorithm>
+#include <cctype>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
@@ -149,3 +152,88 @@ void linspace(double a, double b, int num, double* arr) {
 }
 
 }  // extern "C"
+
+// Matrix Market coordinate reader of the reference drivers
+// (TESTS/COMMON/io.c:115-217): real or integer, general or symmetric; for a
+// symmetric file the stored triangle is followed by the mirrored
+// off-diagonal entries, in file order; indices are shifted by idxout - idxin.
+// Returns 0, 1 for an unsupported or malformed file (as the reference), or
+// EVSL_ERR_ARG when the file cannot be opened (the reference exits).
+extern "C" int evsl_amd_read_coo_mm(const char* path, int idxin, int idxout, cooMat* A) {
+  if (!path || !A) return EVSL_ERR_ARG;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return EVSL_ERR_ARG;
+  struct Closer {
+    FILE* f;
+    ~Closer() { std::fclose(f); }
+  } closer{f};
+  char line[4096];
+  if (!std::fgets(line, sizeof line, f)) return 1;
+  // banner: %%MatrixMarket matrix coordinate <real|integer> <general|symmetric>
+  char tag[64] = {0}, obj[64] = {0}, fmt[64] = {0}, field[64] = {0}, sym[64] = {0};
+  if (std::sscanf(line, "%63s %63s %63s %63s %63s", tag, obj, fmt, field, sym) != 5) return 1;
+  auto lower = [](char* s) {
+    for (; *s; ++s) *s = (char)std::tolower((unsigned char)*s);
+  };
+  lower(obj);
+  lower(fmt);
+  lower(field);
+  lower(sym);
+  if (std::strcmp(tag, "%%MatrixMarket") != 0 || std::strcmp(obj, "matrix") != 0) return 1;
+  if (std::strcmp(fmt, "coordinate") != 0) return 1;
+  if (std::strcmp(field, "real") != 0 && std::strcmp(field, "integer") != 0) return 1;
+  const bool symmetric = std::strcmp(sym, "symmetric") == 0;
+  if (!symmetric && std::strcmp(sym, "general") != 0) return 1;
+  long nrow = -1, ncol = -1, nnz = -1;
+  while (std::fgets(line, sizeof line, f)) {
+    if (line[0] == '%') continue;
+    if (std::sscanf(line, "%ld %ld %ld", &nrow, &ncol, &nnz) == 3) break;
+    nrow = -1;
+  }
+  if (nrow < 0 || ncol < 0 || nnz < 0) return 1;
+  if (nrow != ncol) return 1;
+  const long cap = symmetric ? 2 * nnz : nnz;
+  std::vector<int> ir((size_t)cap), jc((size_t)cap);
+  std::vector<double> vv((size_t)cap);
+  for (long k = 0; k < nnz; ++k) {
+    if (!std::fgets(line, sizeof line, f)) return 1;
+    char* p = line;
+    char* e;
+    const long r = std::strtol(p, &e, 10);
+    if (e == p) return 1;
+    p = e;
+    const long cidx = std::strtol(p, &e, 10);
+    if (e == p) return 1;
+    p = e;
+    const double v = std::strtod(p, &e);
+    if (e == p) return 1;
+    ir[k] = (int)r;
+    jc[k] = (int)cidx;
+    vv[k] = v;
+  }
+  long j = nnz;
+  if (symmetric)
+    for (long k = 0; k < nnz; ++k)
+      if (ir[k] != jc[k]) {
+        ir[j] = jc[k];
+        jc[j] = ir[k];
+        vv[j] = vv[k];
+        ++j;
+      }
+  const int off = idxout - idxin;
+  for (long k = 0; k < j; ++k) {
+    ir[k] += off;
+    jc[k] += off;
+  }
+  A->nrows = (int)nrow;
+  A->ncols = (int)ncol;
+  A->nnz = (int)j;
+  A->ir = (int*)std::malloc((size_t)std::max(j, 1L) * sizeof(int));
+  A->jc = (int*)std::malloc((size_t)std::max(j, 1L) * sizeof(int));
+  A->vv = (double*)std::malloc((size_t)std::max(j, 1L) * sizeof(double));
+  if (!A->ir || !A->jc || !A->vv) return EVSL_ERR_ALLOC;
+  std::memcpy(A->ir, ir.data(), (size_t)j * sizeof(int));
+  std::memcpy(A->jc, jc.data(), (size_t)j * sizeof(int));
+  std::memcpy(A->vv, vv.data(), (size_t)j * sizeof(double));
+  return 0;
+}

@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .types import MVFUNC, PolParams, c_double_p, c_int_p, dptr, iptr, CsrMat
+from .types import MVFUNC, PolParams, c_double_p, c_int_p, dptr, iptr, CsrMat, CooMat, HostCsr
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libevsl_amd.so")
@@ -32,7 +32,7 @@ EXPORTS = [
     "evsl_create_csr_gpu", "evsl_free_csr_gpu", "SetAMatrix_device_csr",
     "set_pol_def", "find_pol", "free_pol",
     "kpmdos", "spslicer", "LanDos", "spslicer2",
-    "ChebLanNr", "ChebLanTr", "LanBounds",
+    "ChebLanNr", "ChebLanTr", "LanBounds", "LanTrbounds",
     "cooMat_to_csrMat", "free_csr", "free_coo", "csr_copy", "arrays_copyto_csrMat",
     "matvec_csr", "rand_double", "rand_double_device", "randn_double",
     "randn_double_device", "sort_double", "linspace", "evsl_timer", "StatsPrint",
@@ -40,7 +40,8 @@ EXPORTS = [
     "evsl_amd_set_filter_precision", "evsl_amd_get_filter_precision",
     "evsl_amd_set_refine_passes", "evsl_amd_set_refine_guard", "evsl_amd_get_stats", "evsl_amd_reset_stats",
     "evsl_amd_last_error", "evsl_amd_stats_size", "kpmdos_ex", "evsl_amd_bench_filter",
-    "ChebLanNr_batch",
+    "ChebLanNr_batch", "evsl_amd_read_coo_mm", "evsl_amd_bench_copy",
+    "evsl_amd_spmm_vec",
 ]
 
 
@@ -49,7 +50,8 @@ class AmdStats(C.Structure):
                 ("t_ritz", C.c_double), ("n_filter_steps", C.c_size_t),
                 ("n_filter_cols", C.c_size_t), ("n_lanczos", C.c_size_t),
                 ("n_reorth_pass", C.c_size_t), ("filter_bytes", C.c_double),
-                ("reorth_bytes", C.c_double), ("n_refine_cols", C.c_size_t)]
+                ("reorth_bytes", C.c_double), ("n_refine_cols", C.c_size_t),
+                ("n_refine_extra", C.c_size_t)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -97,6 +99,11 @@ class EVSL:
             "ChebLanNr": (I, [dp, I, D, dp, pp, ip, ddp, ddp, ddp, VP]),
             "ChebLanTr": (I, [I, I, dp, I, D, dp, pp, ip, ddp, ddp, ddp, VP]),
             "LanBounds": (I, [I, dp, dp, dp]),
+            "LanTrbounds": (I, [I, I, D, dp, I, dp, dp, C.c_void_p]),
+            "evsl_amd_read_coo_mm": (I, [C.c_char_p, I, I, C.POINTER(CooMat)]),
+            "cooMat_to_csrMat": (I, [I, C.POINTER(CooMat), C.POINTER(CsrMat)]),
+            "free_coo": (None, [C.POINTER(CooMat)]),
+            "free_csr": (None, [C.POINTER(CsrMat)]),
             "rand_double": (V, [I, dp]),
             "rand_double_device": (V, [I, dp]),
             "randn_double": (V, [I, dp]),
@@ -111,6 +118,8 @@ class EVSL:
             "evsl_amd_last_error": (C.c_char_p, []),
             "evsl_amd_stats_size": (C.c_size_t, []),
             "evsl_amd_bench_filter": (I, [I, I, I, I, D, D, dp, dp]),
+            "evsl_amd_bench_copy": (I, [C.c_size_t, I, dp]),
+            "evsl_amd_spmm_vec": (I, [I, I, I]),
             "ChebLanNr_batch": (I, [I, dp, I, D, dp, pp, ip, ddp, ddp, ddp, VP]),
         }
         for name, (res, args) in sig.items():
@@ -274,6 +283,32 @@ class EVSL:
         err = self.lib.LanBounds(msteps, dptr(v), C.byref(lmin), C.byref(lmax))
         return err, lmin.value, lmax.value
 
+    def LanTrbounds(self, lanm, maxit, tol, v, bndtype, fstats=None):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        lmin, lmax = C.c_double(), C.c_double()
+        err = self.lib.LanTrbounds(lanm, maxit, tol, dptr(v), bndtype, C.byref(lmin),
+                                   C.byref(lmax), fstats)
+        return err, lmin.value, lmax.value
+
+    def read_mm(self, path, idxin=1):
+        """Matrix Market file -> (err, HostCsr) through the library's reader
+        and cooMat_to_csrMat, the ingest path of TESTS/PLanN/MMPLanN.c:102-112."""
+        coo = CooMat()
+        err = self.lib.evsl_amd_read_coo_mm(str(path).encode(), idxin, 0, C.byref(coo))
+        if err:
+            return err, None
+        csr = CsrMat()
+        err = self.lib.cooMat_to_csrMat(0, C.byref(coo), C.byref(csr))
+        self.lib.free_coo(C.byref(coo))
+        if err:
+            return err, None
+        n, nnz = csr.nrows, csr.nnz
+        ia = np.ctypeslib.as_array(csr.ia, shape=(n + 1,)).copy()
+        ja = np.ctypeslib.as_array(csr.ja, shape=(nnz,)).copy()
+        a = np.ctypeslib.as_array(csr.a, shape=(nnz,)).copy()
+        self.lib.free_csr(C.byref(csr))
+        return 0, HostCsr(n, ia, ja, a)
+
     # ---- extensions -----------------------------------------------------------------
     def set_precision(self, prec):
         return self.lib.evsl_amd_set_filter_precision(prec)
@@ -297,3 +332,12 @@ class EVSL:
         err = self.lib.evsl_amd_bench_filter(B, prec, mode, nsteps, cc, dd, C.byref(ms),
                                              C.byref(by))
         return err, ms.value, by.value
+
+    def bench_copy(self, nbytes, reps=20):
+        """(err, ms per copy) of a device copy of nbytes; 2*nbytes move."""
+        ms = C.c_double()
+        err = self.lib.evsl_amd_bench_copy(nbytes, reps, C.byref(ms))
+        return err, ms.value
+
+    def spmm_vec(self, B, prec, mode):
+        return self.lib.evsl_amd_spmm_vec(B, prec, mode)

@@ -148,6 +148,10 @@ int ChebLanTr(int lanm, int nev, double *intv, int maxit, double tol, double *vi
               polparams *pol, int *nev2, double **vals, double **W, double **resW,
               FILE *fstats);
 int LanBounds(int msteps, double *v, double *lmin, double *lmax);
+/* SRC/lanTrbounds.c:35-421 (INC/evsl.h:193): thick-restart Lanczos bounds;
+ * bndtype <= 1 simple, otherwise Kato-Temple; vinit host or device */
+int LanTrbounds(int lanm, int maxit, double tol, double *vinit, int bndtype,
+                double *lammin, double *lammax, FILE *fstats);
 
 /* sparse helpers (INC/evsl.h:211-237) */
 int cooMat_to_csrMat(int cooidx, cooMat *coo, csrMat *csr);

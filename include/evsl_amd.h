@@ -31,7 +31,8 @@ typedef struct _evslAmdStats {
   size_t n_reorth_pass;  /* Gram-Schmidt passes                           */
   double filter_bytes;   /* algorithmic bytes of the filter launches      */
   double reorth_bytes;   /* algorithmic bytes of the Gram-Schmidt passes  */
-  size_t n_refine_cols;  /* Ritz vectors entering the FP64 refinement     */
+  size_t n_refine_cols;  /* Ritz-vector columns filtered by refinement     */
+  size_t n_refine_extra; /* refinement passes beyond the requested ones   */
 } evslAmdStats;
 
 #ifdef __cplusplus
@@ -41,10 +42,12 @@ extern "C" {
 /* filter precision for ChebLanNr / ChebLanTr / ChebLanNr_batch */
 int evsl_amd_set_filter_precision(int prec);
 int evsl_amd_get_filter_precision(void);
-/* number of FP64 filtered Rayleigh-Ritz refinement passes in MIXED mode */
+/* number of filtered Rayleigh-Ritz refinement passes in MIXED mode (>= 1);
+ * up to two more run while a Ritz value in [a, b] is near convergence */
 int evsl_amd_set_refine_passes(int passes);
 /* guard threshold of the refinement: Ritz values of p(A) down to
- * guard * bar enter the FP64 Rayleigh-Ritz space (0 < guard <= 1) */
+ * guard * bar enter the FP64 Rayleigh-Ritz space (0 < guard <= 1,
+ * default 0.5) */
 int evsl_amd_set_refine_guard(double guard);
 
 int evsl_amd_get_stats(evslAmdStats *s);
@@ -71,14 +74,31 @@ int ChebLanNr_batch(int nslices, double *intvs, int maxit, double tol, double *v
                     polparams *pols, int *nevOut, double **lamo, double **Wo, double **reso,
                     FILE *fstats);
 
+/* Matrix Market coordinate reader of the reference drivers
+ * (TESTS/COMMON/io.c:115-217, read_coo_MM): real/integer, general/symmetric
+ * (the stored triangle, then its mirrored off-diagonal entries); indices
+ * shifted by idxout - idxin; arrays malloc'ed (free_coo).  Returns 0, 1 for
+ * an unsupported or malformed file, EVSL_ERR_ARG if it cannot be opened. */
+int evsl_amd_read_coo_mm(const char *path, int idxin, int idxout, cooMat *A);
+
 /* Measurement hook: time nsteps launches of the fused filter kernel on the
  * current matrix with HIP events: block width B in {1..8,16,32}, mode 1 a
  * ChebAv step of the filter for spectrum centre cc and half-width dd (in
  * EVSL_PREC_MIXED on the shifted FP32 matrix), mode 2 a KPM step (FP64, B a
- * power of two).  Returns the mean launch time and the algorithmic bytes one
- * launch moves. */
+ * power of two), mode 3 a step of the refinement's residual recurrence.
+ * Returns the mean launch time and the algorithmic bytes one launch moves. */
 int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, double dd,
                           double *ms_per_step, double *bytes_per_step);
+
+/* Columns per lane of the fused filter kernel evsl_amd_bench_filter times
+ * for (B, prec, mode): its template instance is
+ * k_cheb_spmm<float|double, B, V, mode, true>; -1 for an unsupported B. */
+int evsl_amd_spmm_vec(int B, int prec, int mode);
+
+/* Measurement hook: mean time of a device copy of `bytes` (16-byte
+ * accesses) over reps launches -- the achievable-bandwidth reference
+ * (2 * bytes move per copy). */
+int evsl_amd_bench_copy(size_t bytes, int reps, double *ms_per_copy);
 
 #ifdef __cplusplus
 }

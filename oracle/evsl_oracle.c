@@ -1081,6 +1081,134 @@ int ora_LanBounds(int msteps, const double *v, double *lmin, double *lmax) {
 }
 
 /* ------------------------------------------------------------------ */
+/* Thick-restart Lanczos spectrum bounds (SRC/lanTrbounds.c:35-421):   */
+/* Lanczos on A with full reorthogonalization, restarted with the two  */
+/* extreme Ritz pairs; simple (|beta s|) or Kato-Temple bounds.        */
+/* ------------------------------------------------------------------ */
+int ora_LanTrbounds(int lanm, int maxit, double tol, const double *vinit, int bndtype,
+                    double *lammin, double *lammax) {
+  const int n = g_n;
+  const size_t nl = n;
+  double lmin = 0.0, lmax = 0.0, t1, t2;
+  lanm = evsl_min(lanm, n);
+  const int lanm1 = lanm + 1;
+  const size_t L1 = lanm1;
+  if (lanm == n) maxit = evsl_min(maxit, n);
+  const int min_inner_step = 5;
+  int it = 0, trlen = 0, i;
+  double *V = malloc(nl * L1 * sizeof(double));
+  double *T = calloc(L1 * L1, sizeof(double));
+  double *Rval = malloc(lanm * sizeof(double));
+  double *Rvec = malloc(2 * nl * sizeof(double));
+  double *EvecT = malloc(L1 * L1 * sizeof(double));
+  double *hw = malloc(L1 * sizeof(double) + nl * sizeof(double));
+  double s[3];
+  memcpy(V, vinit, nl * sizeof(double));
+  scal(n, 1.0 / nrm2(n, V), V);
+  int ret = 0;
+  while (it < maxit) {
+    double wn = 0.0, beta = 0.0;
+    int nwn = 0;
+    int k = trlen > 0 ? trlen + 1 : 0;
+    if (k + min_inner_step > lanm1) { /* the reference exits (:139-143) */
+      ret = 3;
+      break;
+    }
+    if (trlen > 0) { /* thick-restart special step (:145-215) */
+      int k1 = k - 1;
+      double *v = V + (size_t)k1 * nl, *vnew = v + nl;
+      ora_matvec(v, vnew);
+      for (i = 0; i < trlen; i++) {
+        T[i * L1 + i] = Rval[i];
+        wn += fabs(Rval[i]);
+      }
+      s[k1] = dot(n, v, vnew);
+      for (int j = 0; j < k; j++) axpy(n, -s[j], V + (size_t)j * nl, vnew);
+      for (i = 0; i < k1; i++) {
+        T[trlen * L1 + i] = s[i];
+        T[i * L1 + trlen] = s[i];
+        wn += 2.0 * fabs(s[i]);
+      }
+      T[trlen * L1 + trlen] = s[k1];
+      wn += fabs(s[k1]);
+      beta = nrm2(n, vnew);
+      wn += 2.0 * beta;
+      nwn += 3 * k;
+      if (beta * nwn < ORTH_TOL * wn) {
+        ora_rand_double(n, vnew);
+        ora_CGS_DGKS(n, k, NGS_MAX, V, vnew, &beta, hw);
+        scal(n, 1.0 / beta, vnew);
+        beta = 0.0;
+      } else {
+        scal(n, 1.0 / beta, vnew);
+      }
+      T[k1 * L1 + k] = beta;
+      T[k * L1 + k1] = beta;
+    }
+    double *zold = k > 0 ? V + (size_t)(k - 1) * nl : NULL;
+    while (k < lanm && it < maxit) { /* Lanczos inner loop (:222-300) */
+      k++;
+      double *v = V + (size_t)(k - 1) * nl, *vnew = v + nl;
+      ora_matvec(v, vnew);
+      it++;
+      if (zold) axpy(n, -beta, zold, vnew);
+      double alpha = dot(n, v, vnew);
+      T[(k - 1) * L1 + (k - 1)] = alpha;
+      wn += fabs(alpha);
+      axpy(n, -alpha, v, vnew);
+      ora_CGS_DGKS(n, k, NGS_MAX, V, vnew, &beta, hw);
+      wn += 2.0 * beta;
+      nwn += 3;
+      zold = v;
+      if (beta * nwn < ORTH_TOL * wn) {
+        ora_rand_double(n, vnew);
+        ora_CGS_DGKS(n, k, NGS_MAX, V, vnew, &beta, hw);
+        scal(n, 1.0 / beta, vnew);
+        beta = 0.0;
+      } else {
+        scal(n, 1.0 / beta, vnew);
+      }
+      T[k * L1 + (k - 1)] = beta;
+      T[(k - 1) * L1 + k] = beta;
+    }
+    if (ora_sym_eig(k, T, lanm1, EvecT, lanm1, Rval)) {
+      ret = -3;
+      break;
+    }
+    /* spikes: beta times the last components of the extreme eigenvectors */
+    s[0] = beta * EvecT[k - 1];
+    s[1] = beta * EvecT[(size_t)(k - 1) * L1 + (k - 1)];
+    if (bndtype <= 1) {
+      t1 = fabs(s[0]);
+      t2 = fabs(s[1]);
+    } else { /* Kato-Temple */
+      t1 = 2.0 * s[0] * s[0] / (Rval[1] - Rval[0]);
+      t2 = 2.0 * s[1] * s[1] / (Rval[k - 1] - Rval[k - 2]);
+    }
+    lmin = Rval[0] - t1;
+    lmax = Rval[k - 1] + t2;
+    /* the two extreme Ritz vectors become the restart set (:333-357) */
+    gemv_n(n, k, V, nl, EvecT, Rvec);
+    gemv_n(n, k, V, nl, EvecT + (size_t)(k - 1) * L1, Rvec + nl);
+    trlen = 2;
+    memcpy(V, Rvec, 2 * nl * sizeof(double));
+    Rval[1] = Rval[k - 1];
+    if (t1 + t2 < tol * (fabs(lmin) + fabs(lmax))) break;
+    memset(T, 0, L1 * L1 * sizeof(double));
+    memcpy(V + (size_t)trlen * nl, V + (size_t)k * nl, nl * sizeof(double));
+  }
+  *lammin = lmin;
+  *lammax = lmax;
+  free(V);
+  free(T);
+  free(Rval);
+  free(Rvec);
+  free(EvecT);
+  free(hw);
+  return ret;
+}
+
+/* ------------------------------------------------------------------ */
 /* Lanczos DOS (SRC/landos.c:37-196), Simpson (SRC/simpson.c:25-50),   */
 /* slicer on the DOS curve (SRC/spslice2.c:25-49)                      */
 /* ------------------------------------------------------------------ */

@@ -52,6 +52,7 @@ class Oracle:
             "ora_ChebLanNr": (I, [dp, I, D, dp, pp, ip, ddp, ddp, ddp]),
             "ora_ChebLanTr": (I, [I, I, dp, I, D, dp, pp, ip, ddp, ddp, ddp]),
             "ora_LanBounds": (I, [I, dp, dp, dp]),
+            "ora_LanTrbounds": (I, [I, I, D, dp, I, dp, dp]),
             "ora_LanDos": (I, [I, I, I, dp, dp, dp, dp]),
             "ora_simpson": (V, [dp, dp, I]),
             "ora_spslicer2": (V, [dp, dp, I, I, dp]),
@@ -234,6 +235,13 @@ class Oracle:
         v = np.ascontiguousarray(v, dtype=np.float64)
         lmin, lmax = C.c_double(), C.c_double()
         err = self.lib.ora_LanBounds(msteps, dptr(v), C.byref(lmin), C.byref(lmax))
+        return err, lmin.value, lmax.value
+
+    def LanTrbounds(self, lanm, maxit, tol, v, bndtype):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        lmin, lmax = C.c_double(), C.c_double()
+        err = self.lib.ora_LanTrbounds(lanm, maxit, tol, dptr(v), bndtype, C.byref(lmin),
+                                       C.byref(lmax))
         return err, lmin.value, lmax.value
 
     def last_lanczos_steps(self):

@@ -23,6 +23,13 @@ def lib():
     return C.CDLL(LIB_PATH)
 
 
+@pytest.fixture(scope="module")
+def lib_ev(lib):
+    """The ctypes mirror without a started context (host entry points only)."""
+    from evsl_amd.evsl import EVSL
+    return EVSL()
+
+
 def declared_functions():
     names = set()
     for h in ("evsl.h", "evsl_amd.h"):
@@ -138,3 +145,20 @@ def test_host_rng_matches_reference_mapping(lib, libc):
     from tests.conftest import libc_rand_double
     assert np.array_equal(a, libc_rand_double(libc, 1000))
     assert a.min() >= -1.0 and a.max() <= 1.0
+
+
+def test_read_coo_mm_matches_scipy(lib_ev):
+    """The drivers' Matrix Market reader (TESTS/COMMON/io.c:115-217) plus
+    cooMat_to_csrMat on the reference's own MATRICES/A.mtx: the symmetric
+    expansion gives scipy's matrix entry for entry."""
+    import os
+    import scipy.io
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "A.mtx")
+    err, A = lib_ev.read_mm(path)
+    assert err == 0
+    ref = scipy.io.mmread(path).tocsr()
+    ref.sort_indices()
+    assert A.n == ref.shape[0]
+    assert np.array_equal(A.ia, ref.indptr) and np.array_equal(A.ja, ref.indices)
+    assert np.array_equal(A.a, ref.data)
+    assert lib_ev.read_mm(path + ".missing")[0] < 0

@@ -57,8 +57,14 @@ def test_batch_fp64_equals_plain_and_mixed_matches(ev, case):
         assert np.array_equal(lp, lb) and np.array_equal(rp, rb)
     ev.set_precision(PREC_MIXED)
     err, mx = ev.ChebLanNr_batch(ivs, mlan, 1e-8, v0, pols)
+    # the drop-in ChebLanNr in MIXED mode is the one-slice batch: per-column
+    # results do not depend on the block width, so slice 0 is reproduced
+    e1, l1, _, r1 = ev.ChebLanNr(ivs[0], mlan, 1e-8, v0, pols[0])
     ev.set_precision(PREC_FP64)
     assert err == 0, ev.last_error()
+    assert e1 == 0, ev.last_error()
+    assert len(l1) == len(mx[0][0])
+    np.testing.assert_allclose(l1, mx[0][0], rtol=1e-13, atol=1e-15)
     S = A.to_scipy()
     for s, ((e, lp, _, _), (lm, Ym, rm)) in enumerate(zip(plain, mx)):
         assert len(lm) == len(lp), (s, len(lm), len(lp))

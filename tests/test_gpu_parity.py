@@ -274,3 +274,87 @@ def test_full_size_slice_properties(ev):
     assert np.abs(Ys.T @ Ys - np.eye(len(cols))).max() < 1e-7
     R = A.to_scipy() @ Ys - Ys * lam[cols]
     assert np.max(np.linalg.norm(R, axis=0)) < 1e-8
+
+
+def both_solve_tr(ev, oracle, intv, lanm, nev, maxit, tol, v):
+    pg, po = slice_pol(ev, intv), slice_pol(oracle, intv)
+    assert pg.deg == po.deg
+    t0 = time.time()
+    eg, lg, Yg, rg = ev.ChebLanTr(lanm, nev, intv, maxit, tol, v, pg)
+    t1 = time.time()
+    eo, lo, Yo, ro = oracle.ChebLanTr(lanm, nev, intv, maxit, tol, v, po)
+    t2 = time.time()
+    ev.free_pol(pg)
+    oracle.free_pol(po)
+    assert eg == 0, ev.last_error()
+    assert eo == 0
+    print(f"  ChebLanTr {intv[:2]} lanm {lanm} nev {nev}: deg {pg.deg}, gpu {len(lg)} pairs "
+          f"{t1 - t0:.2f}s, oracle {len(lo)} pairs {t2 - t1:.2f}s", flush=True)
+    return (lg, Yg, rg), (lo, Yo, ro)
+
+
+@pytest.mark.parametrize("case", ["driver", "restarts"])
+def test_cheblantr_parity(ev, oracle, libc, case):
+    """Thick-restart Lanczos (SRC/cheblanTr.c) against the oracle and the
+    closed-form Laplacian spectrum.  'driver' uses the reference driver's
+    sizes (TESTS/PLanR/LapPLanR.c:155-159: nev = ev_int + 2,
+    mlan = max(4 nev, 300), max_its = 3 mlan); 'restarts' uses a small basis
+    and nev so that several restarts, locking across restarts and growth of
+    the locked block all happen."""
+    A = M.laplacian(40, 40, 1)
+    intv = np.array([0.4, 0.8, 0.0, 8.0])
+    exact = M.laplacian_eigs(40, 40, 1, intv[0], intv[1])
+    assert ev.set_matrix(A) == 0
+    oracle.set_csr(A)
+    libc.srand(8)
+    v = oracle.rand_double(A.n)
+    if case == "driver":
+        nev = len(exact) + 2
+        lanm = min(max(4 * nev, 300), A.n)
+    else:
+        nev = 4
+        lanm = 60
+    maxit = 3 * lanm if case == "driver" else 4000
+    (lg, Yg, rg), (lo, _, ro) = both_solve_tr(ev, oracle, intv, lanm, nev, maxit, 1e-8, v)
+    assert len(lg) == len(lo) == len(exact), (len(lg), len(lo), len(exact))
+    np.testing.assert_allclose(np.sort(lg), np.sort(lo), rtol=1e-10, atol=1e-13)
+    np.testing.assert_allclose(np.sort(lg), exact, rtol=1e-10)
+    check_pairs(A, lg, Yg, rg, intv, 1e-8)
+
+
+def test_cheblantr_dense_perturbed(ev, oracle, libc):
+    A = M.perturb_diag(M.laplacian(12, 12, 12), 1e-3, seed=5)
+    intv = np.array([1.0, 1.3, 0.0, 12.001])
+    assert ev.set_matrix(A) == 0
+    oracle.set_csr(A)
+    libc.srand(9)
+    v = oracle.rand_double(A.n)
+    dense = np.linalg.eigvalsh(A.to_scipy().toarray())
+    exact = dense[(dense >= intv[0]) & (dense <= intv[1])]
+    nev = len(exact) + 2
+    lanm = max(4 * nev, 300)
+    (lg, Yg, rg), (lo, _, _) = both_solve_tr(ev, oracle, intv, lanm, nev, 3 * lanm, 1e-8, v)
+    assert len(lg) == len(lo) == len(exact)
+    np.testing.assert_allclose(np.sort(lg), exact, rtol=1e-10, atol=1e-12)
+    check_pairs(A, lg, Yg, rg, intv, 1e-8)
+
+
+@pytest.mark.parametrize("bndtype", [1, 2])
+def test_lantrbounds_parity(ev, oracle, libc, bndtype):
+    """Thick-restart Lanczos bounds (SRC/lanTrbounds.c) on the device against
+    the oracle with the same libc rand() start vector: equal to 1e-10
+    relative, and enclosing the dense spectrum."""
+    for A in (M.laplacian(40, 40, 1), M.anderson(10, 16.5, seed=4),
+              M.random_symmetric(2000, 8, 32, seed=9)):
+        assert ev.set_matrix(A) == 0
+        oracle.set_csr(A)
+        libc.srand(3)
+        v = oracle.rand_double(A.n)
+        eg, lg, hg = ev.LanTrbounds(50, 200, 1e-8, v, bndtype)
+        eo, lo, ho = oracle.LanTrbounds(50, 200, 1e-8, v, bndtype)
+        assert eg == 0 and eo == 0, ev.last_error()
+        assert lg == pytest.approx(lo, rel=1e-10, abs=1e-12)
+        assert hg == pytest.approx(ho, rel=1e-10, abs=1e-12)
+        d = np.linalg.eigvalsh(A.to_scipy().toarray())
+        scale = abs(d[0]) + abs(d[-1])
+        assert lg <= d[0] + 1e-12 * scale and hg >= d[-1] - 1e-12 * scale

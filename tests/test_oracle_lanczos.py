@@ -160,3 +160,31 @@ def test_lanbounds_encloses_spectrum(oracle, libc):
     err, lo, hi = oracle.LanBounds(40, np.random.default_rng(0).standard_normal(40))
     evb = np.linalg.eigvalsh(B.to_scipy().toarray())
     assert lo <= evb[0] + 1e-9 and hi >= evb[-1] - 1e-9
+
+
+@pytest.mark.parametrize("bndtype", [1, 2])
+def test_lantrbounds_encloses_and_converges(oracle, libc, bndtype):
+    """Thick-restart bounds (SRC/lanTrbounds.c): the returned interval
+    encloses the spectrum (up to rounding of the converged Ritz values) and
+    its ends lie within the stopping tolerance tol*(|lmin|+|lmax|) of the
+    extreme eigenvalues, for the closed-form 2-D/3-D Laplacian spectra and
+    dense spectra of an Anderson and a random matrix; the driver call of
+    TESTS/PLanN/MMPLanN.c:132 (lanm 50, maxit 200, tol 1e-8)."""
+    cases = [
+        (M.laplacian(40, 40, 1), M.laplacian_eigs(40, 40, 1)),
+        (M.laplacian(12, 12, 12), M.laplacian_eigs(12, 12, 12)),
+        (M.anderson(10, 16.5, seed=4), None),
+        (M.random_symmetric(2000, 8, 32, seed=9), None),
+    ]
+    tol = 1e-8
+    for A, ex in cases:
+        if ex is None:
+            ex = np.linalg.eigvalsh(A.to_scipy().toarray())
+        oracle.set_csr(A)
+        libc.srand(3)
+        v = oracle.rand_double(A.n)
+        err, lo, hi = oracle.LanTrbounds(50, 200, tol, v, bndtype)
+        assert err == 0
+        scale = abs(ex[0]) + abs(ex[-1])
+        assert lo <= ex[0] + 1e-12 * scale and hi >= ex[-1] - 1e-12 * scale
+        assert ex[0] - lo < 2 * tol * scale and hi - ex[-1] < 2 * tol * scale
