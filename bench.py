@@ -122,30 +122,33 @@ def run_step(ev, A, lo, hi, args, form):
 
 def cpu_baseline(A, lo, hi, step, args):
     """Time the CPU oracle on a bounded sample of the step and extrapolate:
-    (a) 20 degrees of the KPM recurrence for one vector, (b) one filtered
-    Lanczos step of slice 0 (two ChebAv applications at its degree),
-    (c) one CGS/DGKS pass against 64 basis vectors.  The full step is then
-    priced from the Lanczos steps and degrees the GPU run took."""
+    (a) 40 degrees of the KPM recurrence for one vector, (b) the first four
+    filtered Lanczos steps of slice 0 (five ChebAv applications at its
+    degree, start-vector filter included), (c) one CGS/DGKS pass against 128
+    basis vectors.  The full step is then priced from the Lanczos steps and
+    degrees the GPU run took."""
     from oracle.pyoracle import Oracle
     orc = Oracle()
     orc.set_csr(A)
     n = A.n
     intv = np.array([args.a, args.b, lo, hi])
+    kdeg = 40
     t0 = orc.timer()
-    orc.kpmdos(20, 1, 1, intv)
-    t_kpm_deg = (orc.timer() - t0) / 20.0
+    orc.kpmdos(kdeg, 1, 1, intv)
+    t_kpm_deg = (orc.timer() - t0) / kdeg
     s0 = step["slices"][0]
     iv = np.array([s0["a"], s0["b"], lo, hi])
     pol = orc.default_pol()
     pol.damping, pol.thresh_int, pol.thresh_ext, pol.max_deg = 2, 0.8, 0.2, 3000
     orc.find_pol(iv, pol)
     v0 = np.random.default_rng(11).uniform(-1.0, 1.0, n)
+    lan_steps = 4
     t0 = orc.timer()
-    orc.ChebLanNr(iv, 1, args.tol, v0, pol)
-    t_lan1 = orc.timer() - t0
+    orc.ChebLanNr(iv, lan_steps, args.tol, v0, pol)
+    t_lan = orc.timer() - t0
     orc.free_pol(pol)
-    t_deg = t_lan1 / (2.0 * s0["deg"])          # two ChebAv calls at this degree
-    k = 64
+    t_deg = t_lan / ((lan_steps + 1.0) * s0["deg"])   # ChebAv calls at this degree
+    k = 128
     Q = np.random.default_rng(0).standard_normal((n, k))
     w = np.random.default_rng(1).standard_normal(n)
     t0 = orc.timer()
@@ -158,13 +161,13 @@ def cpu_baseline(A, lo, hi, step, args):
         steps = s.get("steps", avg_steps)
         est += (steps + 1) * s["deg"] * t_deg
         est += 1.5 * t_col * steps * (steps + 1) / 2.0   # ~1.5 DGKS passes per step
-    sample_secs = t_kpm_deg * 20 + t_lan1 + t_col * k
+    sample_secs = t_kpm_deg * kdeg + t_lan + t_col * k
     total_pairs = sum(s["nev"] for s in step["slices"])
     return {"value": total_pairs / est, "unit": "eigenpairs/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle on the same matrix: 20 KPM degrees, 1 filtered Lanczos step of "
-                       f"slice 0 (deg {s0['deg']}), 1 CGS pass vs 64 vectors "
-                       f"({sample_secs:.1f}s); full step extrapolated to {est:.0f}s from the "
-                       f"GPU run's per-slice Lanczos steps and degrees"),
+            "sample": (f"oracle (C, -O2, 1 thread) on the same matrix: {kdeg} KPM degrees, "
+                       f"{lan_steps} filtered Lanczos steps of slice 0 (deg {s0['deg']}), 1 CGS "
+                       f"pass vs {k} vectors ({sample_secs:.1f}s); full step extrapolated to "
+                       f"{est:.0f}s from the GPU run's per-slice Lanczos steps and degrees"),
             "extrapolated_step_s": est}
 
 

@@ -52,11 +52,11 @@ int block_filter(int B, const FilterOp& op, const double* mus_d, int maxdeg, con
     void* xkp1 = bufs[(k - 1) % 3];
     const void* xk = (k == 1) ? X0 : bufs[(k - 2) % 3];
     const void* xkm1 = (k == 1) ? nullptr : (k == 2 ? X0 : bufs[(k - 3) % 3]);
+    const int yf = evsl_dev::y_flush(k, maxdeg);
     EVSL_HIP(evsl_dev::cheb_spmm(op.A, B, op.f32, evsl_dev::kCheb, xk, xkm1, xkp1, Y, nullptr,
                                  mus_d + (size_t)k * B, mus_d, k == 1 ? op.t1 : op.t2, op.cc,
-                                 k == 1, nullptr, c.stream));
-    // xk (gathered once), xkm1, y in; xkp1, y out (first degree: xk in, xkp1, y out)
-    const int streams = k == 1 ? 3 : 5;
+                                 k == 1, yf, nullptr, c.stream));
+    const int streams = evsl_dev::cheb_streams(yf, k == 1);
     c.astats.filter_bytes += (double)c.A.nnz * (vs + 4) + 4.0 * (c.n + 1) +
                              (double)streams * B * c.n * vs;
   }
@@ -207,7 +207,7 @@ int apply_A_cols(int n, int m, const double* U, double* W, BlockWork& Wk) {
     void* Yb = Wk.buf(1, n, B);
     EVSL_HIP(evsl_dev::pack_cols(n, B, Wk.colptr_in.as<const double*>(), X, false, c.stream));
     EVSL_HIP(evsl_dev::cheb_spmm(A, B, false, evsl_dev::kPlain, X, nullptr, Yb, nullptr, nullptr,
-                                 nullptr, nullptr, 1.0, 0.0, false, nullptr, c.stream));
+                                 nullptr, nullptr, 1.0, 0.0, false, 0, nullptr, c.stream));
     EVSL_HIP(evsl_dev::unpack_cols(n, B, Yb, false, Wk.colptr_out.as<double* const>(), c.stream));
     EVSL_HIP(hipStreamSynchronize(c.stream));
   }
@@ -314,11 +314,13 @@ int residual_filter(int n, int m, double* U, double* W, const polparams* pol, Bl
         void* xkp1 = bufs[(k - 1) % 3];
         const void* xk = (k == 1) ? X0 : bufs[(k - 2) % 3];
         const void* xkm1 = (k == 1) ? nullptr : (k == 2 ? X0 : bufs[(k - 3) % 3]);
+        const int yf = evsl_dev::y_flush(k, deg - 1);
         EVSL_HIP(evsl_dev::cheb_spmm(op.A, B, true, evsl_dev::kRes, xk, xkm1, xkp1, Yb, X0,
                                      g_d.as<double>() + (size_t)k * B, mp_d.as<double>() + 2 * k,
-                                     op.t2, op.cc, k == 1, nullptr, c.stream));
+                                     op.t2, op.cc, k == 1, yf, nullptr, c.stream));
+        // the kCheb streams plus the residual block R
         c.astats.filter_bytes += (double)c.A.nnz * 8 + 4.0 * (n + 1) +
-                                 (double)(k == 1 ? 4 : 6) * B * n * 4;
+                                 (double)(evsl_dev::cheb_streams(yf, k == 1) + 1) * B * n * 4;
       }
       c.astats.n_filter_steps += deg - 1;
       c.astats.n_filter_cols += (size_t)(deg - 1) * B;
@@ -767,6 +769,257 @@ int ChebLanNr_batch(int nslices, double* intvs, int maxit, double tol, double* v
       }
     }
   }
+  return 0;
+}
+
+
+// Polynomial-filtered subspace iteration (SRC/chebsi.c:39-336): a block of
+// nev vectors is filtered with p(A) (one blocked SpMM per degree for up to 32
+// columns), deflated against the locked columns, re-orthonormalised, and
+// every cvcheck = 5 iterations Rayleigh-Ritz on p(A) rotates the block;
+// columns whose p(A)-residual is below tol/100 and whose A-residual is below
+// tol are locked.  The filter runs in FP64.  vinit holds nev vectors (n x nev,
+// column-major, host or device).
+int ChebSI(int nev, double* intv, int maxit, double tol, double* vinit, polparams* pol,
+           int* nevo, double** lamo, double** Yo, double** reso, FILE* fstats) {
+  if (int e = ensure_started()) return e;
+  Context& c = ctx();
+  if (c.op == 0) return EVSL_ERR_NOMAT;
+  const double t_all = evsl_timer();
+  const double tolP = tol * 0.01;
+  const int n = c.n;
+  const size_t nl = n;
+  *nevo = 0;
+  *lamo = *Yo = *reso = nullptr;
+  if (check_intv(intv) < 0) return 0;
+  const double aa = intv[0], bb = intv[1];
+  const int cvcheck = 5, kNbuf = 2;
+  const int deg = pol->deg;
+  const double gamB = pol->gam, bar = pol->bar;
+  if (fstats)
+    std::fprintf(fstats, "Cheb Poly- deg = %d, gam = %.15e, bar: %.15e\n", deg, gamB, bar);
+  if (gamB > 1.0 || gamB < -1.0) {
+    std::fprintf(stdout, "gamB error %.15e\n", gamB);
+    return -1;
+  }
+  if (nev <= 0 || nev > n) {
+    set_error_msg("ChebSI: block size must be in [1, n]");
+    return EVSL_ERR_ARG;
+  }
+  if (fstats) std::fprintf(fstats, "Step 2: ChebSI, block size = %d\n", nev);
+  DevBuf Vb, PVb, Rb, Bb, Tb, Eb, gp, part, s2d, tvd, r2d, wcol, vtmp;
+  const size_t blk = nl * nev * sizeof(double);
+  EVSL_HIP(Vb.alloc(blk));
+  EVSL_HIP(PVb.alloc(blk));
+  EVSL_HIP(Rb.alloc(blk));
+  EVSL_HIP(Bb.alloc(blk));
+  EVSL_HIP(Tb.alloc((size_t)nev * nev * sizeof(double)));
+  EVSL_HIP(Eb.alloc((size_t)nev * nev * sizeof(double)));
+  EVSL_HIP(gp.alloc(evsl_dev::gemm_tn_part_elems(n, nev, nev) * sizeof(double)));
+  const int nch8 = (n + 8191) / 8192;
+  EVSL_HIP(part.alloc((size_t)nch8 * nev * sizeof(double)));
+  EVSL_HIP(s2d.alloc(nev * sizeof(double)));
+  EVSL_HIP(tvd.alloc(nev * sizeof(double)));
+  EVSL_HIP(r2d.alloc(nev * sizeof(double)));
+  EVSL_HIP(wcol.alloc(nl * sizeof(double)));
+  double* V = Vb.as<double>();
+  double* PV = PVb.as<double>();
+  double* R = Rb.as<double>();
+  double* buf = Bb.as<double>();
+  auto col = [&](double* M, int j) { return M + (size_t)j * nl; };
+  LanWork L;
+  L.nch = evsl_dev::gemv_t_chunks(n);
+  EVSL_HIP(L.gt_part.alloc((size_t)L.nch * (nev + 1) * sizeof(double)));
+  EVSL_HIP(L.h.alloc((size_t)(nev + 1) * sizeof(double)));
+  BlockWork bw;
+  std::vector<double> T((size_t)nev * nev), evecT((size_t)nev * nev), evalT(nev), Lam(nev),
+      res(nev);
+  std::vector<int> act_idx(nev), lock_idx(nev);
+  const double* v0;
+  if (int e = as_device_vector(vinit, nl * nev, vtmp, &v0)) return e;
+  // orth (SRC/misc_la.c:317-342): column-by-column CGS/DGKS into Vo
+  auto orth = [&](const double* Vin, int k, double* Vo) -> int {
+    EVSL_HIP(hipMemcpyAsync(Vo, Vin, nl * k * sizeof(double), hipMemcpyDeviceToDevice,
+                            c.stream));
+    double nrm;
+    if (int e = nrm2_dev(n, Vo, &nrm)) return e;
+    EVSL_HIP(evsl_dev::scal(n, 1.0 / nrm, Vo, c.stream));
+    for (int i = 1; i < k; ++i) {
+      double* vi = Vo + (size_t)i * nl;
+      double old;
+      if (int e = nrm2_dev(n, vi, &old)) return e;
+      if (int e = cgs_dgks(n, i, Vo, vi, old * old, &nrm, L)) return e;
+      EVSL_HIP(evsl_dev::scal(n, 1.0 / nrm, vi, c.stream));
+    }
+    return 0;
+  };
+  // PV(:, j0:j0+k) = p(A) V(:, j0:j0+k)
+  auto filter = [&](int j0, int k) -> int {
+    std::vector<const double*> in(k);
+    std::vector<double*> out(k);
+    std::vector<const polparams*> pp(k, pol);
+    for (int j = 0; j < k; ++j) {
+      in[j] = col(V, j0 + j);
+      out[j] = col(PV, j0 + j);
+    }
+    return filter_columns(k, in.data(), out.data(), pp.data(), false, bw);
+  };
+  // X(:, nlock:nlock+nact) -= V(:, 0:nlock) V(:, 0:nlock)^T X(:, nlock:nlock+nact)
+  auto deflate = [&](double* X, int nlock, int nact) -> int {
+    EVSL_HIP(evsl_dev::gemm_tn(n, nlock, nact, V, n, col(X, nlock), n, Tb.as<double>(), nlock,
+                               gp.as<double>(), c.stream));
+    std::vector<double> Th((size_t)nlock * nact);
+    EVSL_HIP(hipMemcpy(Th.data(), Tb.p, Th.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (double& t : Th) t = -t;
+    EVSL_HIP(hipMemcpy(Tb.p, Th.data(), Th.size() * sizeof(double), hipMemcpyHostToDevice));
+    EVSL_HIP(evsl_dev::gemm_nn(n, nact, nlock, V, n, Tb.as<double>(), nlock, col(buf, nlock), n,
+                               c.stream));
+    std::vector<double> ones(nact, 1.0);
+    EVSL_HIP(hipMemcpy(s2d.p, ones.data(), nact * sizeof(double), hipMemcpyHostToDevice));
+    EVSL_HIP(evsl_dev::col_scale_add(n, nact, col(X, nlock), n, s2d.as<double>(),
+                                     col(buf, nlock), n, c.stream));
+    return 0;
+  };
+  if (int e = orth(v0, nev, V)) return e;
+  if (int e = filter(0, nev)) return e;
+  long nmv = (long)deg * nev;
+  int it = 0, nlock = 0, nact = nev, nlock_ab = 0;
+  bool find_more = true;
+  while (it < maxit && find_more) {
+    const size_t nnact = nl * nact;
+    EVSL_HIP(hipMemcpyAsync(col(V, nlock), col(PV, nlock), nnact * sizeof(double),
+                            hipMemcpyDeviceToDevice, c.stream));
+    if (nlock > 0)
+      if (int e = deflate(V, nlock, nact)) return e;
+    if (int e = orth(col(V, nlock), nact, col(buf, nlock))) return e;
+    EVSL_HIP(hipMemcpyAsync(col(V, nlock), col(buf, nlock), nnact * sizeof(double),
+                            hipMemcpyDeviceToDevice, c.stream));
+    if (int e = filter(nlock, nact)) return e;
+    nmv += (long)deg * nact;
+    if (nlock > 0)
+      if (int e = deflate(PV, nlock, nact)) return e;
+    if ((it + 1) % cvcheck == 0 || it + 1 == maxit) {
+      // Rayleigh-Ritz on p(A): T = V_act^T PV_act, eigenpairs, rotate both blocks
+      EVSL_HIP(evsl_dev::gemm_tn(n, nact, nact, col(V, nlock), n, col(PV, nlock), n,
+                                 Tb.as<double>(), nact, gp.as<double>(), c.stream));
+      EVSL_HIP(hipMemcpy(T.data(), Tb.p, (size_t)nact * nact * sizeof(double),
+                         hipMemcpyDeviceToHost));
+      for (int j = 0; j < nact; ++j)
+        for (int i = 0; i < j; ++i) {
+          const double a = 0.5 * (T[(size_t)j * nact + i] + T[(size_t)i * nact + j]);
+          T[(size_t)j * nact + i] = T[(size_t)i * nact + j] = a;
+        }
+      if (int e = sym_eig_host(nact, T.data(), nact, evecT.data(), nact, &evalT[nlock]))
+        return e;
+      EVSL_HIP(hipMemcpy(Eb.p, evecT.data(), (size_t)nact * nact * sizeof(double),
+                         hipMemcpyHostToDevice));
+      EVSL_HIP(evsl_dev::gemm_nn(n, nact, nact, col(V, nlock), n, Eb.as<double>(), nact,
+                                 col(buf, nlock), n, c.stream));
+      EVSL_HIP(hipMemcpyAsync(col(V, nlock), col(buf, nlock), nnact * sizeof(double),
+                              hipMemcpyDeviceToDevice, c.stream));
+      EVSL_HIP(evsl_dev::gemm_nn(n, nact, nact, col(PV, nlock), n, Eb.as<double>(), nact,
+                                 col(buf, nlock), n, c.stream));
+      EVSL_HIP(hipMemcpyAsync(col(PV, nlock), col(buf, nlock), nnact * sizeof(double),
+                              hipMemcpyDeviceToDevice, c.stream));
+      // residual norms with respect to p(A): ||PV_i - evalT_i V_i||
+      EVSL_HIP(hipMemcpy(tvd.p, &evalT[nlock], nact * sizeof(double), hipMemcpyHostToDevice));
+      EVSL_HIP(evsl_dev::col_res2(n, nact, col(PV, nlock), n, col(V, nlock), n,
+                                  tvd.as<double>(), part.as<double>(), r2d.as<double>(),
+                                  c.stream));
+      std::vector<double> rP2(nact);
+      EVSL_HIP(hipMemcpy(rP2.data(), r2d.p, nact * sizeof(double), hipMemcpyDeviceToHost));
+      int nlock_new = 0, nlock_ab_new = 0;
+      const int nact_old = nact;
+      nact = 0;
+      for (int i = nlock; i < nev; ++i) {
+        const double resP = std::sqrt(std::max(rP2[i - nlock], 0.0));
+        bool locked = false;
+        if (resP < tolP) {
+          double* vi = col(V, i);
+          if (int e = apply_A(vi, wcol.as<double>())) return e;
+          ++nmv;
+          EVSL_HIP(evsl_dev::col_dot(n, 1, vi, n, wcol.as<double>(), n, part.as<double>(),
+                                     tvd.as<double>(), c.stream));
+          EVSL_HIP(evsl_dev::col_res2(n, 1, wcol.as<double>(), n, vi, n, tvd.as<double>(),
+                                      part.as<double>(), r2d.as<double>(), c.stream));
+          double rq, rA2;
+          EVSL_HIP(hipMemcpy(&rq, tvd.p, sizeof(double), hipMemcpyDeviceToHost));
+          EVSL_HIP(hipMemcpy(&rA2, r2d.p, sizeof(double), hipMemcpyDeviceToHost));
+          const double resA = std::sqrt(std::max(rA2, 0.0));
+          if (resA < tol) {
+            lock_idx[nlock_new] = i;
+            res[nlock + nlock_new] = resA;
+            Lam[nlock + nlock_new] = rq;
+            ++nlock_new;
+            if (rq >= aa - kEpsM && rq <= bb + kEpsM) ++nlock_ab_new;
+            locked = true;
+          }
+        }
+        if (!locked) act_idx[nact++] = i;
+      }
+      // newly locked columns first, then the active ones (via buf and R)
+      for (int i = 0; i < nlock_new; ++i) {
+        EVSL_HIP(hipMemcpyAsync(col(buf, nlock + i), col(V, lock_idx[i]), nl * sizeof(double),
+                                hipMemcpyDeviceToDevice, c.stream));
+        EVSL_HIP(hipMemcpyAsync(col(R, nlock + i), col(PV, lock_idx[i]), nl * sizeof(double),
+                                hipMemcpyDeviceToDevice, c.stream));
+      }
+      for (int i = 0; i < nact; ++i) {
+        EVSL_HIP(hipMemcpyAsync(col(buf, nlock + nlock_new + i), col(V, act_idx[i]),
+                                nl * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+        EVSL_HIP(hipMemcpyAsync(col(R, nlock + nlock_new + i), col(PV, act_idx[i]),
+                                nl * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+      }
+      EVSL_HIP(hipMemcpyAsync(col(V, nlock), col(buf, nlock), nl * nact_old * sizeof(double),
+                              hipMemcpyDeviceToDevice, c.stream));
+      EVSL_HIP(hipMemcpyAsync(col(PV, nlock), col(R, nlock), nl * nact_old * sizeof(double),
+                              hipMemcpyDeviceToDevice, c.stream));
+      nlock += nlock_new;
+      nlock_ab += nlock_ab_new;
+      if (fstats)
+        std::fprintf(fstats, "it %4d:   nMV %7ld,  nlock %3d, nlock_ab  %3d\n", it + 1, nmv,
+                     nlock, nlock_ab);
+      // stop when NBUF locked pairs lie outside [a, b] or the whole block is locked
+      if (nlock - nlock_ab >= kNbuf || nlock == nev) {
+        find_more = false;
+        if (fstats)
+          std::fprintf(fstats, "-------------------------------------\n No ev.s left to be "
+                               "computed\n Number of evals found = %d\n", nlock_ab);
+      }
+    }
+    ++it;
+  }
+  // keep the locked pairs inside [a, b], in locking order
+  double* lam_o = (double*)std::malloc((size_t)nev * sizeof(double));
+  double* res_o = (double*)std::malloc((size_t)nev * sizeof(double));
+  double* Y_o = (double*)_evsl_Malloc_device(blk);
+  if (!lam_o || !res_o || !Y_o) {
+    std::free(lam_o);
+    std::free(res_o);
+    if (Y_o) _evsl_Free_device(Y_o);
+    return EVSL_ERR_ALLOC;
+  }
+  int idx = 0;
+  for (int i = 0; i < nlock; ++i) {
+    const double t = Lam[i];
+    if (t >= aa - kEpsM && t <= bb + kEpsM) {
+      lam_o[idx] = t;
+      res_o[idx] = res[i];
+      EVSL_HIP(hipMemcpyAsync(Y_o + (size_t)idx * nl, col(V, i), nl * sizeof(double),
+                              hipMemcpyDeviceToDevice, c.stream));
+      ++idx;
+    }
+  }
+  EVSL_HIP(hipStreamSynchronize(c.stream));
+  *nevo = idx;
+  *lamo = lam_o;
+  *Yo = Y_o;
+  *reso = res_o;
+  const double tall = evsl_timer() - t_all;
+  if (fstats)
+    std::fprintf(fstats, "------This slice consumed: \nMatvecs :        %ld\ntotal  time :    %.2f\n",
+                 nmv, tall);
+  evslstat.t_iter = tall;
   return 0;
 }
 

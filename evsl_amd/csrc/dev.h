@@ -42,16 +42,36 @@ enum Mode { kPlain = 0, kCheb = 1, kKpm = 2, kRes = 3 };
 //   acc = A * xk
 //   kPlain: xkp1 = acc
 //   kCheb : w = t*(acc - cc*xk) - (xkm1 ? xkm1 : 0); xkp1 = w;
-//           y = (first ? mu0[c]*xk : y) + s[c]*w
+//           y accumulates the last yflush terms of sum_k mu_k T_k:
+//             first (yflush 1): y = mu0[c]*xk + s[c]*w
+//             yflush 3: y += s[c-2B]*xkm1, then += s[c-B]*xk, then += s[c]*w
+//             yflush 2: y += s[c-B]*xk, then += s[c]*w;  1: y += s[c]*w
+//             yflush 0: y untouched (neither read nor written)
 //   kKpm  : w as kCheb; xkp1 = w; part[blk*B + c] = sum_rows w*v0
 //   kRes  : w = t*(acc - cc*xk) - xkm1 + s[c]*v0 (v0 = residual block R);
-//           xkp1 = w; y = (first ? mu0[1]*xk : y) + mu0[0]*w
-// s, mu0: per-column coefficients (device, B entries each; kRes see above).
+//           xkp1 = w; y as kCheb with coefficients mu0[0] (this degree),
+//           mu0[-2], mu0[-4] (the two before) and mu0[1] for xk when first
+// s, mu0: per-column coefficients (device; kRes see above).
 // fp32 selects float vectors and matrix values (float accumulation).
 hipError_t cheb_spmm(const Csr& A, int B, bool fp32, Mode mode, const void* xk,
                      const void* xkm1, void* xkp1, void* y, const void* v0,
                      const double* s, const double* mu0, double t, double cc,
-                     bool first, double* part, hipStream_t st);
+                     bool first, int yflush, double* part, hipStream_t st);
+
+// y terms flushed by launch k of a degree-m series (k = 1..m): the first
+// launch sets y, then every third launch adds three terms, and the last adds
+// what is pending
+inline int y_flush(int k, int m) {
+  if (k == 1) return 1;
+  const int r = (k - 1) % 3;
+  if (r == 0) return 3;
+  return k == m ? r : 0;
+}
+
+// algorithmic vector streams of one kCheb launch (kRes: one more, the
+// residual block): xk (gathered once), xkm1, xkp1, and y in and out when
+// it is flushed; the first launch reads xk and writes xkp1 and y
+inline int cheb_streams(int yflush, bool first) { return first ? 3 : 3 + (yflush ? 2 : 0); }
 
 // dst = src for bytes / 16 16-byte words (the bandwidth reference kernel)
 hipError_t stream_copy(size_t bytes, const void* src, void* dst, hipStream_t st);

@@ -159,4 +159,24 @@ struct ChebWork {
 // y = p(A) v (device vectors) with pol; FP64 or FP32 per ctx().filter_prec
 int cheb_av(const polparams* pol, const double* v, double* y, ChebWork& w, int prec);
 
+// ---- Lanczos building blocks (lanczos.cpp) ----------------------------------
+// check_intv of INC/internal_header.h:194-217: -1 empty interval, -2 outside
+// [lmin, lmax], 0 valid
+int check_intv(const double* intv);
+// device pointer for n doubles the caller holds on the host or the device
+// (a host array is copied into tmp)
+int as_device_vector(const double* p, size_t n, DevBuf& tmp, const double** out);
+// workspace of one filtered Lanczos run
+struct LanWork {
+  DevBuf V, gt_part, h, vrand;
+  ChebWork cw;
+  int nch = 0;
+};
+// CGS with the DGKS test, two GEMV passes at most (SRC/misc_la.c:221-261),
+// against the k columns of Q (column-major, ld n); on entry old2 = ||v||^2;
+// returns ||v|| in *nrm.  L.gt_part and L.h must hold k + 1 columns.
+int cgs_dgks(int n, int k, const double* Q, double* v, double old2, double* nrm, LanWork& L);
+// ||x|| (one device->host read)
+int nrm2_dev(int n, const double* x, double* out);
+
 }  // namespace evsl

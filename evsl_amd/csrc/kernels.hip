@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
     const int2* __restrict__ rbp, int nblk, const VT* __restrict__ xk,
     const VT* __restrict__ xkm1, VT* __restrict__ xkp1, VT* __restrict__ y,
     const VT* __restrict__ v0, const double* __restrict__ scol,
-    const double* __restrict__ mu0col, double t_, double cc_, int first,
+    const double* __restrict__ mu0col, double t_, double cc_, int first, int yflush,
     double* __restrict__ part) {
   static_assert(B % V == 0, "V must divide B");
   constexpr int L = B / V;
@@ -95,23 +95,32 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
   const bool act = tid < NG * L;
   const VT t = (VT)t_, cc = (VT)cc_;
   // kRes: v0 holds the residual block R and scol the per-column factors
-  // 2*T_k(theta_c); mu0col[0..1] = (mu_k, mu_{k-1}) shared by all columns
-  VT s[V], mu0[V], g[V];
+  // 2*T_k(theta_c); mu0col[0..1] = (mu_k, mu_{k-1}) shared by all columns.
+  // y accumulates the last yflush terms T_{k-1}, T_k, T_{k+1} of the series
+  // (0: y untouched); s, s1, s2 are their coefficients.  kCheb keeps the
+  // per-degree coefficient rows B apart (scol - B, scol - 2B), kRes the
+  // (mu_k, mu_1) pairs 2 apart (mu0col - 2, mu0col - 4).
+  VT s[V], s1[V], s2[V], mu0[V], g[V];
 #pragma unroll
   for (int j = 0; j < V; ++j) {
-    s[j] = 0;
+    s[j] = s1[j] = s2[j] = 0;
     mu0[j] = 0;
     g[j] = 0;
     if (MODE == kCheb) {
       s[j] = (VT)scol[c0 + j];
+      if (yflush >= 2) s1[j] = (VT)scol[c0 + j - B];
+      if (yflush >= 3) s2[j] = (VT)scol[c0 + j - 2 * B];
       if (first) mu0[j] = (VT)mu0col[c0 + j];
     } else if (MODE == kRes) {
       g[j] = (VT)scol[c0 + j];
       s[j] = (VT)mu0col[0];
+      if (yflush >= 2) s1[j] = (VT)mu0col[-2];
+      if (yflush >= 3) s2[j] = (VT)mu0col[-4];
       if (first) mu0[j] = (VT)mu0col[1];
     }
   }
-  const bool need_xr = first || MODE == kKpm || cc_ != 0.0;
+  const bool ymode = MODE == kCheb || MODE == kRes;
+  const bool need_xr = first || MODE == kKpm || cc_ != 0.0 || (ymode && yflush >= 2);
   double dacc[V];
 #pragma unroll
   for (int j = 0; j < V; ++j) dacc[j] = 0.0;
@@ -127,7 +136,7 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
     if (MODE == kPlain) return o;
     if (need_xr) o.xr = ld(xk, i);
     if (xkm1) o.km1 = ld(xkm1, i);
-    if ((MODE == kCheb || MODE == kRes) && !first) o.yo = ld(y, i);
+    if (ymode && yflush > 0 && !first) o.yo = ld(y, i);
     if (MODE == kRes) o.rr = ld(v0, i);
     if (MODE == kKpm) o.v = ld(v0, i);
     return o;
@@ -146,14 +155,19 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
       w.v[j] = wj;
     }
     *reinterpret_cast<Vec*>(xkp1 + i) = w;
-    if (MODE == kCheb || MODE == kRes) {
-      Vec yn;
+    if (ymode) {
+      if (yflush > 0) {
+        // the same additions, in the same order, as one update per degree
+        Vec yn;
 #pragma unroll
-      for (int j = 0; j < V; ++j) {
-        const VT yo = first ? mu0[j] * o.xr.v[j] : o.yo.v[j];
-        yn.v[j] = yo + s[j] * w.v[j];
+        for (int j = 0; j < V; ++j) {
+          VT yv = first ? mu0[j] * o.xr.v[j] : o.yo.v[j];
+          if (yflush >= 3) yv = yv + s2[j] * o.km1.v[j];
+          if (yflush >= 2) yv = yv + s1[j] * o.xr.v[j];
+          yn.v[j] = yv + s[j] * w.v[j];
+        }
+        *reinterpret_cast<Vec*>(y + i) = yn;
       }
-      *reinterpret_cast<Vec*>(y + i) = yn;
     } else if (MODE == kKpm) {
 #pragma unroll
       for (int j = 0; j < V; ++j) dacc[j] += (double)w.v[j] * (double)o.v.v[j];
@@ -882,7 +896,7 @@ inline unsigned grid_for(size_t n, int per_block, unsigned cap) {
 template <typename VT, int B, int V, int MODE, bool LDS>
 hipError_t launch_variant(const Csr& A, const void* xk, const void* xkm1, void* xkp1, void* y,
                           const void* v0, const double* s, const double* mu0, double t,
-                          double cc, bool first, double* part, hipStream_t st) {
+                          double cc, bool first, int yflush, double* part, hipStream_t st) {
   const VT* av;
   if constexpr (sizeof(VT) == 4)
     av = A.a32;
@@ -895,7 +909,7 @@ hipError_t launch_variant(const Csr& A, const void* xk, const void* xkm1, void* 
   const int2* rb = reinterpret_cast<const int2*>(A.rb[slot]);
   hipLaunchKernelGGL((k_cheb_spmm<VT, B, V, MODE, LDS>), dim3(nblk), dim3(kWG), 0, st, A.ia,
                      A.ja, av, rb, nblk, (const VT*)xk, (const VT*)xkm1, (VT*)xkp1, (VT*)y,
-                     (const VT*)v0, s, mu0, t, cc, first ? 1 : 0, part);
+                     (const VT*)v0, s, mu0, t, cc, first ? 1 : 0, yflush, part);
   return hipGetLastError();
 }
 
@@ -922,12 +936,13 @@ struct SpmmPolicy {
 template <typename VT, int MODE>
 hipError_t launch_cheb_B(const Csr& A, int B, const void* xk, const void* xkm1, void* xkp1,
                          void* y, const void* v0, const double* s, const double* mu0,
-                         double t, double cc, bool first, double* part, hipStream_t st) {
+                         double t, double cc, bool first, int yflush, double* part,
+                         hipStream_t st) {
 #define EVSL_CHEB_CASE(BB)                                                                 \
   case BB:                                                                                 \
     return launch_variant<VT, BB, SpmmPolicy<VT, BB, MODE>::V, MODE,                      \
                           SpmmPolicy<VT, BB, MODE>::LDS>(A, xk, xkm1, xkp1, y, v0, s, mu0, t, \
-                                                         cc, first, part, st);
+                                                         cc, first, yflush, part, st);
   if constexpr (MODE == kKpm) {
     switch (B) {
       EVSL_CHEB_CASE(1)
@@ -1022,30 +1037,32 @@ int cheb_spmm_nblk(const Csr& A, int B, bool fp32, Mode mode) {
 
 hipError_t cheb_spmm(const Csr& A, int B, bool fp32, Mode mode, const void* xk,
                      const void* xkm1, void* xkp1, void* y, const void* v0, const double* s,
-                     const double* mu0, double t, double cc, bool first, double* part,
-                     hipStream_t st) {
+                     const double* mu0, double t, double cc, bool first, int yflush,
+                     double* part, hipStream_t st) {
   if (A.n <= 0) return hipSuccess;
+  if (yflush < 0 || yflush > 3 || (first && yflush != 1) || (yflush == 3 && !xkm1))
+    return hipErrorInvalidValue;
   if (fp32) {
     switch (mode) {
       case kPlain:
-        return launch_cheb_B<float, kPlain>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
+        return launch_cheb_B<float, kPlain>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, yflush, part, st);
       case kCheb:
-        return launch_cheb_B<float, kCheb>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
+        return launch_cheb_B<float, kCheb>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, yflush, part, st);
       case kKpm:
-        return launch_cheb_B<float, kKpm>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
+        return launch_cheb_B<float, kKpm>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, yflush, part, st);
       case kRes:
-        return launch_cheb_B<float, kRes>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
+        return launch_cheb_B<float, kRes>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, yflush, part, st);
     }
   } else {
     switch (mode) {
       case kPlain:
-        return launch_cheb_B<double, kPlain>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
+        return launch_cheb_B<double, kPlain>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, yflush, part, st);
       case kCheb:
-        return launch_cheb_B<double, kCheb>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
+        return launch_cheb_B<double, kCheb>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, yflush, part, st);
       case kKpm:
-        return launch_cheb_B<double, kKpm>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
+        return launch_cheb_B<double, kKpm>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, yflush, part, st);
       case kRes:
-        return launch_cheb_B<double, kRes>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, part, st);
+        return launch_cheb_B<double, kRes>(A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc, first, yflush, part, st);
     }
   }
   return hipErrorInvalidValue;

@@ -77,7 +77,7 @@ int kpm_impl(int Mdeg, int damping, int nvec, const double* intv, int kind,
         const void* xk = (k == 0) ? x0 : buf[(k - 1) % 3];
         const void* xkm1 = (k == 0) ? nullptr : (k == 1 ? x0 : buf[(k - 2) % 3]);
         EVSL_HIP(evsl_dev::cheb_spmm(A, B, false, evsl_dev::kKpm, xk, xkm1, xkp1, nullptr, x0,
-                                     nullptr, nullptr, sc, ctr, false, red_part(), c.stream));
+                                     nullptr, nullptr, sc, ctr, false, 0, red_part(), c.stream));
         EVSL_HIP(evsl_dev::sum_block_partials(red_part(), nblk_kpm, B,
                                               dots.as<double>() + (size_t)(k + 1) * B,
                                               red_scratch(), c.stream));
@@ -251,23 +251,30 @@ int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, doub
   const size_t nb = (size_t)n * B;
   DevBuf X, coef;
   EVSL_HIP(X.alloc(5 * nb * vs));
-  EVSL_HIP(coef.alloc(2 * 64 * sizeof(double)));
+  EVSL_HIP(coef.alloc(256 * sizeof(double)));
   char* base = X.as<char>();
   EVSL_HIP(hipMemsetAsync(base, 0, 5 * nb * vs, c.stream));
-  EVSL_HIP(evsl_dev::fill(2 * 64, 0.01, coef.as<double>(), c.stream));
+  EVSL_HIP(evsl_dev::fill(256, 0.01, coef.as<double>(), c.stream));
+  // coefficient rows with room for the look-back of a 3-term y flush
+  const double* scol = coef.as<double>() + 64;
+  const double* mu0col = coef.as<double>() + 160;
   void* buf[5];
   for (int i = 0; i < 5; ++i) buf[i] = base + i * nb * vs;
   const evsl_dev::Mode md =
       mode == 2 ? evsl_dev::kKpm : (mode == 3 ? evsl_dev::kRes : evsl_dev::kCheb);
+  // steady state of a long series: y is flushed (3 terms) every third launch
+  nsteps = (nsteps + 2) / 3 * 3;
   hipEvent_t e0, e1;
   EVSL_HIP(hipEventCreate(&e0));
   EVSL_HIP(hipEventCreate(&e1));
-  for (int it = 0; it < nsteps + 2; ++it) {
-    if (it == 2) EVSL_HIP(hipEventRecord(e0, c.stream));
+  int flushes = 0;
+  for (int it = 0; it < nsteps + 3; ++it) {
+    if (it == 3) EVSL_HIP(hipEventRecord(e0, c.stream));
+    const int yf = (mode == 2 || it % 3 != 2) ? 0 : 3;
+    if (it >= 3 && yf) ++flushes;
     EVSL_HIP(evsl_dev::cheb_spmm(op.A, B, f32, md, buf[it % 3], buf[(it + 1) % 3],
-                                 buf[(it + 2) % 3], buf[3], buf[4], coef.as<double>(),
-                                 coef.as<double>() + 64, op.t2, mode == 2 ? cc : op.cc, false,
-                                 red_part(), c.stream));
+                                 buf[(it + 2) % 3], buf[3], buf[4], scol, mu0col, op.t2,
+                                 mode == 2 ? cc : op.cc, false, yf, red_part(), c.stream));
   }
   EVSL_HIP(hipEventRecord(e1, c.stream));
   EVSL_HIP(hipEventSynchronize(e1));
@@ -276,10 +283,15 @@ int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, doub
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *ms_per_step = ms / nsteps;
-  // KPM: xk (gathered once), xkm1, v0 in, xkp1 out; ChebAv: xk, xkm1, y in,
-  // xkp1, y out; residual recurrence: ChebAv's plus the residual block R
-  const int streams = mode == 2 ? 4 : (mode == 3 ? 6 : 5);
-  *bytes_per_step = (double)c.A.nnz * (vs + 4) + 4.0 * (n + 1) + (double)streams * nb * vs;
+  // per launch: KPM xk (gathered once), xkm1, v0 in, xkp1 out; ChebAv xk,
+  // xkm1 in, xkp1 out, plus y in and out on a flush; the residual
+  // recurrence adds the residual block R
+  double streams;
+  if (mode == 2)
+    streams = 4.0;
+  else
+    streams = 3.0 + 2.0 * flushes / nsteps + (mode == 3 ? 1.0 : 0.0);
+  *bytes_per_step = (double)c.A.nnz * (vs + 4) + 4.0 * (n + 1) + streams * nb * vs;
   return 0;
 }
 

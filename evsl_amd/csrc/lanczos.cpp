@@ -26,14 +26,16 @@ constexpr double kOrthTol = 1e-14;  // INC/internal_header.h:21
 constexpr int kNgsMax = 2;          // INC/internal_header.h:24
 const double kEpsM = EVSL_DBL_EPS_MULT * DBL_EPSILON;
 
+
+}  // namespace
+
 int check_intv(const double* intv) {
   if (intv[0] >= intv[1]) return -1;
   if (intv[0] >= intv[3] || intv[1] <= intv[2]) return -2;
   return 0;
 }
 
-// device pointer for a caller vector that may live on the host or device
-int as_device_vector(const double* p, int n, DevBuf& tmp, const double** out) {
+int as_device_vector(const double* p, size_t n, DevBuf& tmp, const double** out) {
   hipPointerAttribute_t at;
   hipError_t e = hipPointerGetAttributes(&at, p);
   if (e == hipSuccess && (at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged)) {
@@ -41,11 +43,13 @@ int as_device_vector(const double* p, int n, DevBuf& tmp, const double** out) {
     return 0;
   }
   (void)hipGetLastError();  // clear the error of an unregistered host pointer
-  EVSL_HIP(tmp.alloc((size_t)n * sizeof(double)));
-  EVSL_HIP(hipMemcpy(tmp.p, p, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  EVSL_HIP(tmp.alloc(n * sizeof(double)));
+  EVSL_HIP(hipMemcpy(tmp.p, p, n * sizeof(double), hipMemcpyHostToDevice));
   *out = tmp.as<double>();
   return 0;
 }
+
+namespace {
 
 // HIP-event phase timing accumulated into the extension statistics
 struct EventClock {
@@ -122,11 +126,12 @@ int cheb_av(const polparams* pol, const double* v, double* y, ChebWork& w, int p
       void* xkp1 = buf[(k - 1) % 3];
       const void* xk = (k == 1) ? vin : buf[(k - 2) % 3];
       const void* xkm1 = (k == 1) ? nullptr : (k == 2 ? vin : buf[(k - 3) % 3]);
+      const int yf = evsl_dev::y_flush(k, m);
       EVSL_HIP(evsl_dev::cheb_spmm(op.A, 1, f32, evsl_dev::kCheb, xk, xkm1, xkp1, yout, nullptr,
-                                   mu_d + k, mu_d, k == 1 ? op.t1 : op.t2, op.cc, k == 1,
+                                   mu_d + k, mu_d, k == 1 ? op.t1 : op.t2, op.cc, k == 1, yf,
                                    nullptr, c.stream));
       c.astats.filter_bytes +=
-          filter_step_bytes(c, 1, (int)vs, (int)vs, k == 1 ? 3 : 5);
+          filter_step_bytes(c, 1, (int)vs, (int)vs, evsl_dev::cheb_streams(yf, k == 1));
     }
     if (f32) EVSL_HIP(evsl_dev::f32_to_f64(n, (const float*)yout, y, c.stream));
     c.astats.n_filter_steps += m;
@@ -155,15 +160,6 @@ int cheb_av(const polparams* pol, const double* v, double* y, ChebWork& w, int p
   c.astats.n_filter_cols += m;
   return 0;
 }
-
-namespace {
-
-// Workspace of one filtered Lanczos run
-struct LanWork {
-  DevBuf V, gt_part, h, vrand;
-  ChebWork cw;
-  int nch = 0;
-};
 
 // CGS with the DGKS test, two GEMV passes at most (SRC/misc_la.c:221-261).
 // On entry *old2 = ||v||^2 (already known); returns ||v|| in *nrm.
@@ -198,7 +194,6 @@ int nrm2_dev(int n, const double* x, double* out) {
   return 0;
 }
 
-}  // namespace
 
 // Ritz extraction shared by the filtered Lanczos solvers: given the
 // tridiagonal T_kdim (dT, eT) and the basis V, compute Ritz pairs of A for

@@ -132,8 +132,8 @@ int apply_A(const double* x, double* y) {
   Context& c = g_ctx;
   if (c.op == 1) {
     EVSL_HIP(evsl_dev::cheb_spmm(c.A.view(), 1, false, evsl_dev::kPlain, x, nullptr, y,
-                                 nullptr, nullptr, nullptr, nullptr, 1.0, 0.0, false, nullptr,
-                                 c.stream));
+                                 nullptr, nullptr, nullptr, nullptr, 1.0, 0.0, false, 0,
+                                 nullptr, c.stream));
     return 0;
   }
   if (c.op == 2) {

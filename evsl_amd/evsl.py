@@ -32,7 +32,7 @@ EXPORTS = [
     "evsl_create_csr_gpu", "evsl_free_csr_gpu", "SetAMatrix_device_csr",
     "set_pol_def", "find_pol", "free_pol",
     "kpmdos", "spslicer", "LanDos", "spslicer2",
-    "ChebLanNr", "ChebLanTr", "LanBounds", "LanTrbounds",
+    "ChebLanNr", "ChebLanTr", "LanBounds", "LanTrbounds", "ChebSI",
     "cooMat_to_csrMat", "free_csr", "free_coo", "csr_copy", "arrays_copyto_csrMat",
     "matvec_csr", "rand_double", "rand_double_device", "randn_double",
     "randn_double_device", "sort_double", "linspace", "evsl_timer", "StatsPrint",
@@ -98,6 +98,7 @@ class EVSL:
             "LanDos": (I, [I, I, I, dp, dp, dp, dp]),
             "ChebLanNr": (I, [dp, I, D, dp, pp, ip, ddp, ddp, ddp, VP]),
             "ChebLanTr": (I, [I, I, dp, I, D, dp, pp, ip, ddp, ddp, ddp, VP]),
+            "ChebSI": (I, [I, dp, I, D, dp, pp, ip, ddp, ddp, ddp, VP]),
             "LanBounds": (I, [I, dp, dp, dp]),
             "LanTrbounds": (I, [I, I, D, dp, I, dp, dp, C.c_void_p]),
             "evsl_amd_read_coo_mm": (I, [C.c_char_p, I, I, C.POINTER(CooMat)]),
@@ -256,6 +257,20 @@ class EVSL:
         if err:
             return err, None, None, None
         return (0,) + self._collect(len(vinit), nev2.value, lam, W, res)
+
+    def ChebSI(self, nev, intv, maxit, tol, V0, pol, fstats=None):
+        """V0: n x nev start block (host array, column-major = V0.T row-major)."""
+        intv = np.ascontiguousarray(intv, dtype=np.float64)
+        V0 = np.asfortranarray(V0, dtype=np.float64)
+        n = V0.shape[0]
+        nevo = C.c_int()
+        lam, W, res = c_double_p(), c_double_p(), c_double_p()
+        err = self.lib.ChebSI(nev, dptr(intv), maxit, tol,
+                              V0.ctypes.data_as(c_double_p), C.byref(pol), C.byref(nevo),
+                              C.byref(lam), C.byref(W), C.byref(res), fstats)
+        if err:
+            return err, None, None, None
+        return (0,) + self._collect(n, nevo.value, lam, W, res)
 
     def ChebLanNr_batch(self, intvs, maxit, tol, vinit, pols, n=None, fstats=None):
         """All slices of one matrix in one batched run.  intvs: list of

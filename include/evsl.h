@@ -148,6 +148,10 @@ int ChebLanTr(int lanm, int nev, double *intv, int maxit, double tol, double *vi
               polparams *pol, int *nev2, double **vals, double **W, double **resW,
               FILE *fstats);
 int LanBounds(int msteps, double *v, double *lmin, double *lmax);
+/* SRC/chebsi.c:39-336 (INC/evsl.h:154): filtered subspace iteration on a
+ * block of nev vectors vinit (n x nev, host or device) */
+int ChebSI(int nev, double *intv, int maxit, double tol, double *vinit, polparams *pol,
+           int *nevo, double **lamo, double **Yo, double **reso, FILE *fstats);
 /* SRC/lanTrbounds.c:35-421 (INC/evsl.h:193): thick-restart Lanczos bounds;
  * bndtype <= 1 simple, otherwise Kato-Temple; vinit host or device */
 int LanTrbounds(int lanm, int maxit, double tol, double *vinit, int bndtype,

@@ -1209,6 +1209,139 @@ int ora_LanTrbounds(int lanm, int maxit, double tol, const double *vinit, int bn
 }
 
 /* ------------------------------------------------------------------ */
+/* Filtered subspace iteration (SRC/chebsi.c:39-336) and the block     */
+/* orthonormalisation it uses (orth, SRC/misc_la.c:317-342).           */
+/* ------------------------------------------------------------------ */
+static void ora_orth(const double *V, int n, int k, double *Vo, double *hw) {
+  const size_t nl = n;
+  memcpy(Vo, V, nl * k * sizeof(double));
+  double nrmv = nrm2(n, Vo);
+  scal(n, 1.0 / nrmv, Vo);
+  for (int i = 1; i < k; i++) {
+    ora_CGS_DGKS(n, i, NGS_MAX, Vo, Vo + (size_t)i * nl, &nrmv, hw);
+    scal(n, 1.0 / nrmv, Vo + (size_t)i * nl);
+  }
+}
+
+/* T (m x p, ld m) = A(:, 0:m)^T B(:, 0:p), columns of length n */
+static void gemm_tn_(int n, int m, int p, const double *A, const double *B, double *T) {
+  for (int j = 0; j < p; j++)
+    for (int i = 0; i < m; i++) T[(size_t)j * m + i] = dot(n, A + (size_t)i * n, B + (size_t)j * n);
+}
+
+int ora_ChebSI(int nev, const double *intv, int maxit, double tol, const double *vinit,
+               const polparams *pol, int *nevo, double **lamo, double **Yo, double **reso) {
+  const int n = g_n;
+  const size_t nl = n;
+  const double tolP = tol * 0.01;
+  *nevo = 0;
+  *lamo = *Yo = *reso = NULL;
+  if (check_intv(intv) < 0) return 0;
+  const double aa = intv[0], bb = intv[1];
+  const int cvcheck = 5, nbuf = 2;
+  if (pol->gam > 1.0 || pol->gam < -1.0) return -1;
+  double *T = malloc((size_t)nev * nev * sizeof(double));
+  double *evecT = malloc((size_t)nev * nev * sizeof(double));
+  double *evalT = malloc(nev * sizeof(double));
+  double *V = malloc(nl * nev * sizeof(double)), *PV = malloc(nl * nev * sizeof(double));
+  double *R = malloc(nl * nev * sizeof(double)), *buf = malloc(nl * nev * sizeof(double));
+  double *Lam = malloc(nev * sizeof(double)), *res = malloc(nev * sizeof(double));
+  int *act_idx = malloc(nev * sizeof(int)), *lock_idx = malloc(nev * sizeof(int));
+  double *work = malloc(3 * nl * sizeof(double));
+  double *hw = malloc((nev + 1) * sizeof(double) + nl * sizeof(double));
+  int nlock = 0, nact = nev, nlock_ab = 0, it = 0, find_more = 1;
+  ora_orth(vinit, n, nev, V, hw);
+  for (int j = 0; j < nev; j++) ora_ChebAv(pol, V + (size_t)j * nl, PV + (size_t)j * nl, work);
+  while (it < maxit && find_more) {
+    const size_t nnlock = nl * nlock, nnact = nl * nact;
+    memcpy(V + nnlock, PV + nnlock, nnact * sizeof(double));
+    if (nlock > 0) { /* V_act -= V_lock (V_lock^T V_act) */
+      gemm_tn_(n, nlock, nact, V, V + nnlock, T);
+      for (int j = 0; j < nact; j++)
+        for (int i = 0; i < nlock; i++)
+          axpy(n, -T[(size_t)j * nlock + i], V + (size_t)i * nl, V + nnlock + (size_t)j * nl);
+    }
+    ora_orth(V + nnlock, n, nact, buf, hw);
+    memcpy(V + nnlock, buf, nnact * sizeof(double));
+    for (int j = nlock; j < nlock + nact; j++)
+      ora_ChebAv(pol, V + (size_t)j * nl, PV + (size_t)j * nl, work);
+    if (nlock > 0) { /* PV_act -= V_lock (V_lock^T PV_act) */
+      gemm_tn_(n, nlock, nact, V, PV + nnlock, T);
+      for (int j = 0; j < nact; j++)
+        for (int i = 0; i < nlock; i++)
+          axpy(n, -T[(size_t)j * nlock + i], V + (size_t)i * nl, PV + nnlock + (size_t)j * nl);
+    }
+    if ((it + 1) % cvcheck == 0 || it + 1 == maxit) {
+      gemm_tn_(n, nact, nact, V + nnlock, PV + nnlock, T);
+      if (ora_sym_eig(nact, T, nact, evecT, nact, evalT + nlock)) return -3;
+      for (int j = 0; j < nact; j++) gemv_n(n, nact, V + nnlock, nl, evecT + (size_t)j * nact, buf + nnlock + (size_t)j * nl);
+      memcpy(V + nnlock, buf + nnlock, nnact * sizeof(double));
+      for (int j = 0; j < nact; j++) gemv_n(n, nact, PV + nnlock, nl, evecT + (size_t)j * nact, buf + nnlock + (size_t)j * nl);
+      memcpy(PV + nnlock, buf + nnlock, nnact * sizeof(double));
+      for (int i = nlock; i < nev; i++) {
+        memcpy(R + (size_t)i * nl, PV + (size_t)i * nl, nl * sizeof(double));
+        axpy(n, -evalT[i], V + (size_t)i * nl, R + (size_t)i * nl);
+      }
+      int nlock_new = 0, nlock_ab_new = 0;
+      nact = 0;
+      for (int i = nlock; i < nev; i++) {
+        double resP = nrm2(n, R + (size_t)i * nl);
+        int locked = 0;
+        if (resP < tolP) {
+          ora_matvec(V + (size_t)i * nl, buf);
+          double rq = dot(n, V + (size_t)i * nl, buf);
+          memcpy(R + (size_t)i * nl, buf, nl * sizeof(double));
+          axpy(n, -rq, V + (size_t)i * nl, R + (size_t)i * nl);
+          double resA = nrm2(n, R + (size_t)i * nl);
+          if (resA < tol) {
+            lock_idx[nlock_new] = i;
+            res[nlock + nlock_new] = resA;
+            Lam[nlock + nlock_new] = rq;
+            nlock_new++;
+            if (rq >= aa - EPSM && rq <= bb + EPSM) nlock_ab_new++;
+            locked = 1;
+          }
+        }
+        if (!locked) act_idx[nact++] = i;
+      }
+      for (int i = 0; i < nlock_new; i++) {
+        memcpy(buf + nnlock + (size_t)i * nl, V + (size_t)lock_idx[i] * nl, nl * sizeof(double));
+        memcpy(R + nnlock + (size_t)i * nl, PV + (size_t)lock_idx[i] * nl, nl * sizeof(double));
+      }
+      for (int i = 0; i < nact; i++) {
+        memcpy(buf + nnlock + (size_t)(nlock_new + i) * nl, V + (size_t)act_idx[i] * nl, nl * sizeof(double));
+        memcpy(R + nnlock + (size_t)(nlock_new + i) * nl, PV + (size_t)act_idx[i] * nl, nl * sizeof(double));
+      }
+      memcpy(V + nnlock, buf + nnlock, nnact * sizeof(double));
+      memcpy(PV + nnlock, R + nnlock, nnact * sizeof(double));
+      nlock += nlock_new;
+      nlock_ab += nlock_ab_new;
+      if ((nlock - nlock_ab) >= nbuf || nlock == nev) find_more = 0;
+    }
+    it++;
+  }
+  double *lam_o = malloc(nev * sizeof(double)), *res_o = malloc(nev * sizeof(double));
+  double *Y_o = malloc(nl * nev * sizeof(double));
+  int idx = 0;
+  for (int i = 0; i < nlock; i++) {
+    const double t = Lam[i];
+    if (t >= aa - EPSM && t <= bb + EPSM) {
+      lam_o[idx] = t;
+      res_o[idx] = res[i];
+      memcpy(Y_o + (size_t)idx * nl, V + (size_t)i * nl, nl * sizeof(double));
+      idx++;
+    }
+  }
+  *nevo = idx;
+  *lamo = lam_o;
+  *Yo = Y_o;
+  *reso = res_o;
+  free(T); free(evecT); free(evalT); free(V); free(PV); free(R); free(buf);
+  free(Lam); free(res); free(act_idx); free(lock_idx); free(work); free(hw);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
 /* Lanczos DOS (SRC/landos.c:37-196), Simpson (SRC/simpson.c:25-50),   */
 /* slicer on the DOS curve (SRC/spslice2.c:25-49)                      */
 /* ------------------------------------------------------------------ */

@@ -60,6 +60,8 @@ int ora_ChebLanTr(int lanm, int nev, const double *intv, int maxit, double tol,
                   const double *vinit, const polparams *pol, int *nev2,
                   double **vals, double **W, double **resW);
 int ora_LanBounds(int msteps, const double *v, double *lmin, double *lmax);
+int ora_ChebSI(int nev, const double *intv, int maxit, double tol, const double *vinit,
+               const polparams *pol, int *nevo, double **lamo, double **Yo, double **reso);
 int ora_LanTrbounds(int lanm, int maxit, double tol, const double *vinit, int bndtype,
                     double *lammin, double *lammax);
 int ora_LanDos(int nvec, int msteps, int npts, double *xdos, double *ydos,

@@ -50,6 +50,7 @@ class Oracle:
             "ora_tridiag_eig": (I, [I, dp, dp, dp, dp]),
             "ora_sym_eig": (I, [I, dp, I, dp, I, dp]),
             "ora_ChebLanNr": (I, [dp, I, D, dp, pp, ip, ddp, ddp, ddp]),
+            "ora_ChebSI": (I, [I, dp, I, D, dp, pp, ip, ddp, ddp, ddp]),
             "ora_ChebLanTr": (I, [I, I, dp, I, D, dp, pp, ip, ddp, ddp, ddp]),
             "ora_LanBounds": (I, [I, dp, dp, dp]),
             "ora_LanTrbounds": (I, [I, I, D, dp, I, dp, dp]),
@@ -218,6 +219,19 @@ class Oracle:
         if err:
             return err, None, None, None
         return (0,) + self._take(len(vinit), nev.value, lam, W, res)
+
+    def ChebSI(self, nev, intv, maxit, tol, V0, pol):
+        """V0: n x nev start block."""
+        intv = np.ascontiguousarray(intv, dtype=np.float64)
+        V0 = np.asfortranarray(V0, dtype=np.float64)
+        nevo = C.c_int()
+        lam, W, res = c_double_p(), c_double_p(), c_double_p()
+        err = self.lib.ora_ChebSI(nev, dptr(intv), maxit, tol, V0.ctypes.data_as(c_double_p),
+                                  C.byref(pol), C.byref(nevo), C.byref(lam), C.byref(W),
+                                  C.byref(res))
+        if err:
+            return err, None, None, None
+        return (0,) + self._take(V0.shape[0], nevo.value, lam, W, res)
 
     def ChebLanTr(self, lanm, nev, intv, maxit, tol, vinit, pol):
         intv = np.ascontiguousarray(intv, dtype=np.float64)

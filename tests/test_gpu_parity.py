@@ -358,3 +358,41 @@ def test_lantrbounds_parity(ev, oracle, libc, bndtype):
         d = np.linalg.eigvalsh(A.to_scipy().toarray())
         scale = abs(d[0]) + abs(d[-1])
         assert lg <= d[0] + 1e-12 * scale and hg >= d[-1] - 1e-12 * scale
+
+
+@pytest.mark.parametrize("case", ["lap2d", "perturbed3d"])
+def test_chebsi_parity(ev, oracle, libc, case):
+    """Filtered subspace iteration (SRC/chebsi.c) on the device against the
+    oracle from the same libc rand() start block, with the PSI driver's
+    settings (TESTS/PSI/LapPSI.c:150-177): identical counts, eigenvalues to
+    1e-10 of each other and of the exact spectrum, residuals below tol."""
+    if case == "lap2d":
+        A, intv = M.laplacian(60, 60, 1), np.array([0.4, 0.8, 0.0, 8.0])
+    else:
+        A, intv = M.perturb_diag(M.laplacian(12, 12, 12), 1e-3, seed=2), \
+            np.array([1.0, 1.3, 0.0, 12.001])
+    assert ev.set_matrix(A) == 0
+    oracle.set_csr(A)
+    d = np.linalg.eigvalsh(A.to_scipy().toarray())
+    exact = d[(d >= intv[0]) & (d <= intv[1])]
+    nev = int(1.2 * (1 + len(exact)))
+    pg, po = ev.default_pol(), oracle.default_pol()
+    for p in (pg, po):
+        p.damping, p.thresh_int = 0, 0.5
+    assert ev.find_pol(intv, pg) == 0 and oracle.find_pol(intv, po) == 0
+    libc.srand(4)
+    V0 = oracle.rand_double(A.n * nev).reshape(nev, A.n).T
+    t0 = time.time()
+    eg, lg, Yg, rg = ev.ChebSI(nev, intv, 1000, 1e-8, V0, pg)
+    t1 = time.time()
+    eo, lo, _, ro = oracle.ChebSI(nev, intv, 1000, 1e-8, V0, po)
+    t2 = time.time()
+    ev.free_pol(pg)
+    oracle.free_pol(po)
+    assert eg == 0 and eo == 0, ev.last_error()
+    print(f"  ChebSI {case}: nev {nev} deg {pg.deg}, gpu {len(lg)} pairs {t1 - t0:.2f}s, "
+          f"oracle {len(lo)} pairs {t2 - t1:.2f}s", flush=True)
+    assert len(lg) == len(lo) == len(exact)
+    np.testing.assert_allclose(np.sort(lg), np.sort(lo), rtol=1e-10, atol=1e-13)
+    np.testing.assert_allclose(np.sort(lg), exact, rtol=1e-10, atol=1e-12)
+    check_pairs(A, lg, Yg, rg, intv, 1e-8)

@@ -188,3 +188,31 @@ def test_lantrbounds_encloses_and_converges(oracle, libc, bndtype):
         scale = abs(ex[0]) + abs(ex[-1])
         assert lo <= ex[0] + 1e-12 * scale and hi >= ex[-1] - 1e-12 * scale
         assert ex[0] - lo < 2 * tol * scale and hi - ex[-1] < 2 * tol * scale
+
+
+def test_chebsi_closed_form_and_dense(oracle, libc):
+    """Filtered subspace iteration (SRC/chebsi.c) with the PSI driver's
+    filter settings (TESTS/PSI/LapPSI.c:150-168: damping 0, thresh_int 0.5,
+    nev = 1.2 (1 + count)): every eigenvalue in the interval, against the
+    closed-form 2-D Laplacian spectrum and a dense spectrum."""
+    cases = [(M.laplacian(30, 30, 1), np.array([0.4, 0.8, 0.0, 8.0]), None),
+             (M.perturb_diag(M.laplacian(9, 9, 9), 1e-3, seed=2), np.array([1.0, 1.4, 0.0, 12.001]),
+              None)]
+    for A, intv, _ in cases:
+        oracle.set_csr(A)
+        d = np.linalg.eigvalsh(A.to_scipy().toarray())
+        exact = d[(d >= intv[0]) & (d <= intv[1])]
+        nev = int(1.2 * (1 + len(exact)))
+        pol = oracle.default_pol()
+        pol.damping, pol.thresh_int = 0, 0.5
+        assert oracle.find_pol(intv, pol) == 0
+        libc.srand(4)
+        V0 = oracle.rand_double(A.n * nev).reshape(nev, A.n).T
+        err, lam, Y, res = oracle.ChebSI(nev, intv, 1000, 1e-8, V0, pol)
+        oracle.free_pol(pol)
+        assert err == 0
+        assert len(lam) == len(exact)
+        np.testing.assert_allclose(np.sort(lam), exact, rtol=1e-10, atol=1e-12)
+        assert np.all(res < 1e-8)
+        R = A.to_scipy() @ Y - Y * lam
+        np.testing.assert_allclose(np.linalg.norm(R, axis=0), res, rtol=1e-5, atol=1e-13)
