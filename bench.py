@@ -279,6 +279,9 @@ def main():
                    "n": A.n, "nnz": A.nnz, "form": args.form, "parallelism": "single GPU"},
         "eigenpairs": pairs // args.steps,
         "per_slice_counts": [s["nev"] for s in last["slices"]],
+        "per_slice_degrees": [s["deg"] for s in last["slices"]],
+        "lanczos_steps": int(last["stats"]["n_lanczos"]),
+        "mlan": last["mlan"],
         "max_residual": max(s["max_res"] for s in last["slices"]),
         "kpm_s": last["t_kpm"], "solve_s": last["t_solve"],
         "filter_GBps_in_solver": (st["filter_bytes"] / st["t_filter"] / 1e9
@@ -295,6 +298,7 @@ def main():
         pc = [s["nev"] for s in plain["slices"]]
         pv = sum(pc) / plain["wall"]
         rec["fp64_plain"] = {"value": pv, "wall_s": plain["wall"], "per_slice_counts": pc,
+                             "per_slice_steps": [s["steps"] for s in plain["slices"]],
                              "max_residual": max(s["max_res"] for s in plain["slices"])}
         rec["speedup_vs_fp64_plain"] = value / pv
         rec["counts_match_fp64_plain"] = pc == rec["per_slice_counts"]

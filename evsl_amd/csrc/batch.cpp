@@ -319,11 +319,9 @@ int residual_filter(int n, int m, double* U, double* W, const polparams* pol, Bl
                                      g_d.as<double>() + (size_t)k * B, mp_d.as<double>() + 2 * k,
                                      op.t2, op.cc, k == 1, yf, nullptr, c.stream));
         // the kCheb streams plus the residual block R
-        c.astats.filter_bytes += (double)c.A.nnz * 8 + 4.0 * (n + 1) +
+        c.astats.refine_bytes += (double)c.A.nnz * 8 + 4.0 * (n + 1) +
                                  (double)(evsl_dev::cheb_streams(yf, k == 1) + 1) * B * n * 4;
       }
-      c.astats.n_filter_steps += deg - 1;
-      c.astats.n_filter_cols += (size_t)(deg - 1) * B;
       EVSL_HIP(evsl_dev::unpack_cols(n, B, Yb, true, bw.colptr_out.as<double* const>(), c.stream));
     }
     EVSL_HIP(hipStreamSynchronize(c.stream));

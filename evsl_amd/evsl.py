@@ -51,7 +51,7 @@ class AmdStats(C.Structure):
                 ("n_filter_cols", C.c_size_t), ("n_lanczos", C.c_size_t),
                 ("n_reorth_pass", C.c_size_t), ("filter_bytes", C.c_double),
                 ("reorth_bytes", C.c_double), ("n_refine_cols", C.c_size_t),
-                ("n_refine_extra", C.c_size_t)]
+                ("n_refine_extra", C.c_size_t), ("refine_bytes", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -29,10 +29,11 @@ typedef struct _evslAmdStats {
   size_t n_filter_cols;  /* sum over those launches of the block width    */
   size_t n_lanczos;      /* Lanczos steps                                 */
   size_t n_reorth_pass;  /* Gram-Schmidt passes                           */
-  double filter_bytes;   /* algorithmic bytes of the filter launches      */
+  double filter_bytes;   /* algorithmic bytes of the Lanczos/KPM filter   */
   double reorth_bytes;   /* algorithmic bytes of the Gram-Schmidt passes  */
   size_t n_refine_cols;  /* Ritz-vector columns filtered by refinement     */
   size_t n_refine_extra; /* refinement passes beyond the requested ones   */
+  double refine_bytes;   /* algorithmic bytes of the refinement's filter  */
 } evslAmdStats;
 
 #ifdef __cplusplus
