@@ -215,8 +215,7 @@ def main():
         for tag, (b, pr, mode) in {"filter": (B, prec, 1), "refine": (32, prec, 3),
                                    "kpm": (32, PREC_FP64, 2)}.items():
             err, ms, by = ev.bench_filter(b, pr, mode, 50, cc, dd)
-            vt = "float" if pr == PREC_MIXED and mode != 2 else "double"
-            out[tag] = {"kernel": f"k_cheb_spmm<{vt}, {b}, {ev.spmm_vec(b, pr, mode)}, {mode}, true>",
+            out[tag] = {"kernel": ev.spmm_kernel_name(b, pr, mode),
                         "launch_ms": ms, "bytes_per_launch": by,
                         "achieved_GBps": by / (ms * 1e-3) / 1e9}
         nbytes = 512 << 20
@@ -253,12 +252,12 @@ def main():
 
     # dominant kernel of the timed form: the fused Chebyshev-step SpMM
     if args.form == "mixed":
-        kB, kprec, vt = args.nslices, PREC_MIXED, "float"
+        kB, kprec = args.nslices, PREC_MIXED
     elif args.form == "batch64":
-        kB, kprec, vt = args.nslices, PREC_FP64, "double"
+        kB, kprec = args.nslices, PREC_FP64
     else:
-        kB, kprec, vt = 1, PREC_FP64, "double"
-    kname = f"k_cheb_spmm<{vt}, {kB}, {ev.spmm_vec(kB, kprec, 1)}, 1, true>"
+        kB, kprec = 1, PREC_FP64
+    kname = ev.spmm_kernel_name(kB, kprec, 1)
     err, ms, by = ev.bench_filter(kB, kprec, 1, 200, cc, dd)
     achieved = by / (ms * 1e-3) / 1e9
     err1, ms1, by1 = ev.bench_filter(1, PREC_FP64, 1, 200, cc, dd)

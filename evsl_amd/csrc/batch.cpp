@@ -57,7 +57,7 @@ int block_filter(int B, const FilterOp& op, const double* mus_d, int maxdeg, con
                                  mus_d + (size_t)k * B, mus_d, k == 1 ? op.t1 : op.t2, op.cc,
                                  k == 1, yf, nullptr, c.stream));
     const int streams = evsl_dev::cheb_streams(yf, k == 1);
-    c.astats.filter_bytes += (double)c.A.nnz * (vs + 4) + 4.0 * (c.n + 1) +
+    c.astats.filter_bytes += (double)c.A.nnz * (vs + c.A.idx_bytes()) + 4.0 * (c.n + 1) +
                              (double)streams * B * c.n * vs;
   }
   c.astats.n_filter_steps += maxdeg;
@@ -319,7 +319,7 @@ int residual_filter(int n, int m, double* U, double* W, const polparams* pol, Bl
                                      g_d.as<double>() + (size_t)k * B, mp_d.as<double>() + 2 * k,
                                      op.t2, op.cc, k == 1, yf, nullptr, c.stream));
         // the kCheb streams plus the residual block R
-        c.astats.refine_bytes += (double)c.A.nnz * 8 + 4.0 * (n + 1) +
+        c.astats.refine_bytes += (double)c.A.nnz * (4 + c.A.idx_bytes()) + 4.0 * (n + 1) +
                                  (double)(evsl_dev::cheb_streams(yf, k == 1) + 1) * B * n * 4;
       }
       EVSL_HIP(evsl_dev::unpack_cols(n, B, Yb, true, bw.colptr_out.as<double* const>(), c.stream));

@@ -28,6 +28,9 @@ struct Csr {
   int n = 0, nnz = 0;
   const int* ia = nullptr;
   const int* ja = nullptr;
+  // 16-bit column offsets ja - row, when every row's columns lie within
+  // +-32767 of it (the filter kernels then read these instead of ja)
+  const short* jd = nullptr;
   const double* a64 = nullptr;
   const float* a32 = nullptr;
   // per block width: nblk+1 pairs (first row of the block, its row pointer)

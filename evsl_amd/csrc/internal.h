@@ -50,6 +50,7 @@ struct DevBuf {
 struct DevMatrix {
   int n = 0, nnz = 0;
   int *ia = nullptr, *ja = nullptr;
+  short* jd = nullptr;             // 16-bit column offsets (when they fit)
   double* a64 = nullptr;
   float* a32 = nullptr;            // FP32 copy of the values (unshifted)
   float* a32s = nullptr;           // FP32 copy of (A - shift_cc I) * shift_scale
@@ -58,6 +59,8 @@ struct DevMatrix {
   int* rb[evsl_dev::kNumWidths] = {};
   int nblk[evsl_dev::kNumWidths] = {};
   bool owns_csr = false;  // ia/ja/a64 allocated here (SetAMatrix)
+  // bytes per nonzero of the column index the filter kernels read
+  int idx_bytes() const { return jd ? 2 : 4; }
   evsl_dev::Csr view() const;
   void release();
 };

@@ -63,9 +63,10 @@ struct alignas(sizeof(VT) * V) VecT {
 // (the L lanes of a row load the same address).  The per-column summation
 // order is the same for every (V, LDS), so results are bitwise independent
 // of the variant (the KPM block partials aside, whose tree depends on NG).
-template <typename VT, int B, int V, int MODE, bool LDS>
+template <typename VT, int B, int V, int MODE, bool LDS, bool I16>
 __global__ __launch_bounds__(kWG) void k_cheb_spmm(
-    const int* __restrict__ ia, const int* __restrict__ ja, const VT* __restrict__ av,
+    const int* __restrict__ ia, const int* __restrict__ ja, const short* __restrict__ jd,
+    const VT* __restrict__ av,
     const int2* __restrict__ rbp, int nblk, const VT* __restrict__ xk,
     const VT* __restrict__ xkm1, VT* __restrict__ xkp1, VT* __restrict__ y,
     const VT* __restrict__ v0, const double* __restrict__ scol,
@@ -78,11 +79,18 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
   using Vec = VecT<VT, V>;
   // staging area for ja/a, reused for the group reductions once the row
   // walk is over
-  constexpr size_t kStage = LDS ? (size_t)kNnzCap * (sizeof(int) + sizeof(VT)) : 0;
+  // column indices: int32, or (I16) 16-bit offsets from the row, ja = row + jd
+  using IT = typename std::conditional<I16, short, int>::type;
+  const IT* __restrict__ jx;
+  if constexpr (I16)
+    jx = jd;
+  else
+    jx = ja;
+  constexpr size_t kStage = LDS ? (size_t)kNnzCap * (sizeof(IT) + sizeof(VT)) : 0;
   constexpr size_t kRed = (size_t)kWG * V * sizeof(double);
   __shared__ __align__(16) unsigned char smem[kStage > kRed ? kStage : kRed];
-  int* s_ja = reinterpret_cast<int*>(smem);
-  VT* s_a = reinterpret_cast<VT*>(smem + (size_t)kNnzCap * sizeof(int));
+  IT* s_ja = reinterpret_cast<IT*>(smem);
+  VT* s_a = reinterpret_cast<VT*>(smem + (size_t)kNnzCap * sizeof(IT));
   double* s_red = reinterpret_cast<double*>(smem);
 
   const int blk = xcd_remap(blockIdx.x, nblk);
@@ -185,11 +193,11 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
       qf1 = ia[rf + 1] - p0;
       of = prefetch(rf);
     }
-    const int* jsrc = ja + p0;
+    const IT* jsrc = jx + p0;
     const VT* asrc = av + p0;
     if (LDS) {
       for (int p = tid; p < nz; p += kWG) {
-        s_ja[p] = ja[p0 + p];
+        s_ja[p] = jx[p0 + p];
         s_a[p] = av[p0 + p];
       }
       __syncthreads();
@@ -216,7 +224,7 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const bool ok = q + u < q1;
-          jj[u] = ok ? jsrc[q + u] : 0;
+          jj[u] = ok ? (I16 ? r : 0) + (int)jsrc[q + u] : 0;
           aa[u] = ok ? asrc[q + u] : (VT)0;
         }
 #pragma unroll
@@ -243,7 +251,7 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
     for (int j = 0; j < V; ++j) acc[j] = 0;
     for (int p = p0 + grp; act && p < p1; p += NG) {
       const VT a = av[p];
-      const Vec x = ld(xk, (size_t)ja[p] * B + c0);
+      const Vec x = ld(xk, (size_t)((I16 ? r0 : 0) + (int)jx[p]) * B + c0);
 #pragma unroll
       for (int j = 0; j < V; ++j) acc[j] += a * x.v[j];
     }
@@ -907,9 +915,16 @@ hipError_t launch_variant(const Csr& A, const void* xk, const void* xkm1, void* 
   if (slot < 0 || !A.rb[slot]) return hipErrorInvalidValue;
   const int nblk = A.nblk[slot];
   const int2* rb = reinterpret_cast<const int2*>(A.rb[slot]);
-  hipLaunchKernelGGL((k_cheb_spmm<VT, B, V, MODE, LDS>), dim3(nblk), dim3(kWG), 0, st, A.ia,
-                     A.ja, av, rb, nblk, (const VT*)xk, (const VT*)xkm1, (VT*)xkp1, (VT*)y,
-                     (const VT*)v0, s, mu0, t, cc, first ? 1 : 0, yflush, part);
+  if (A.jd)
+    hipLaunchKernelGGL((k_cheb_spmm<VT, B, V, MODE, LDS, true>), dim3(nblk), dim3(kWG), 0, st,
+                       A.ia, A.ja, A.jd, av, rb, nblk, (const VT*)xk, (const VT*)xkm1,
+                       (VT*)xkp1, (VT*)y, (const VT*)v0, s, mu0, t, cc, first ? 1 : 0, yflush,
+                       part);
+  else
+    hipLaunchKernelGGL((k_cheb_spmm<VT, B, V, MODE, LDS, false>), dim3(nblk), dim3(kWG), 0, st,
+                       A.ia, A.ja, A.jd, av, rb, nblk, (const VT*)xk, (const VT*)xkm1,
+                       (VT*)xkp1, (VT*)y, (const VT*)v0, s, mu0, t, cc, first ? 1 : 0, yflush,
+                       part);
   return hipGetLastError();
 }
 

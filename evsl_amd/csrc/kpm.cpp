@@ -81,7 +81,7 @@ int kpm_impl(int Mdeg, int damping, int nvec, const double* intv, int kind,
         EVSL_HIP(evsl_dev::sum_block_partials(red_part(), nblk_kpm, B,
                                               dots.as<double>() + (size_t)(k + 1) * B,
                                               red_scratch(), c.stream));
-        c.astats.filter_bytes += (double)c.A.nnz * 12 + 4.0 * (n + 1) +
+        c.astats.filter_bytes += (double)c.A.nnz * (8 + c.A.idx_bytes()) + 4.0 * (n + 1) +
                                  (double)(k == 0 ? 3 : 4) * nb * sizeof(double);
       }
       c.astats.n_filter_steps += Mdeg;
@@ -291,8 +291,14 @@ int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, doub
     streams = 4.0;
   else
     streams = 3.0 + 2.0 * flushes / nsteps + (mode == 3 ? 1.0 : 0.0);
-  *bytes_per_step = (double)c.A.nnz * (vs + 4) + 4.0 * (n + 1) + streams * nb * vs;
+  *bytes_per_step =
+      (double)c.A.nnz * (vs + c.A.idx_bytes()) + 4.0 * (n + 1) + streams * nb * vs;
   return 0;
+}
+
+int evsl_amd_spmm_index_bytes(void) {
+  const Context& c = ctx();
+  return c.op == 1 ? c.A.idx_bytes() : 0;
 }
 
 int evsl_amd_spmm_vec(int B, int prec, int mode) {

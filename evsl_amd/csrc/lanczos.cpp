@@ -77,7 +77,8 @@ struct EventClock {
 };
 
 double filter_step_bytes(const Context& c, int B, int sv, int sa, int streams) {
-  return (double)c.A.nnz * (sa + 4) + 4.0 * (c.n + 1) + (double)streams * B * c.n * sv;
+  return (double)c.A.nnz * (sa + c.A.idx_bytes()) + 4.0 * (c.n + 1) +
+         (double)streams * B * c.n * sv;
 }
 
 }  // namespace

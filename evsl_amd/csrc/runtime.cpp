@@ -53,6 +53,7 @@ evsl_dev::Csr DevMatrix::view() const {
   c.nnz = nnz;
   c.ia = ia;
   c.ja = ja;
+  c.jd = jd;
   c.a64 = a64;
   c.a32 = a32;
   for (int w = 0; w < evsl_dev::kNumWidths; ++w) {
@@ -70,6 +71,7 @@ void DevMatrix::release() {
   }
   for (int w = 0; w < evsl_dev::kNumWidths; ++w)
     if (rb[w]) (void)hipFree(rb[w]);
+  if (jd) (void)hipFree(jd);
   if (a32) (void)hipFree(a32);
   if (a32s) (void)hipFree(a32s);
   *this = DevMatrix();
@@ -180,6 +182,26 @@ static int register_device_csr(int n, int nnz, int* ia_d, int* ja_d, double* a_d
     full = has;
   }
   c.A.full_diag = full;
+  // 16-bit column offsets when every |ja - row| fits (banded and stencil
+  // matrices in a local ordering): 2 bytes less per nonzero and filter step
+  {
+    bool fits = true;
+    for (int r = 0; r < n && fits; ++r)
+      for (int k = ia_h[r]; k < ia_h[r + 1]; ++k) {
+        const long d = (long)ja_h[k] - r;
+        if (d < -32768 || d > 32767) {
+          fits = false;
+          break;
+        }
+      }
+    if (fits && nnz > 0) {
+      std::vector<short> jd((size_t)nnz);
+      for (int r = 0; r < n; ++r)
+        for (int k = ia_h[r]; k < ia_h[r + 1]; ++k) jd[k] = (short)(ja_h[k] - r);
+      EVSL_HIP(hipMalloc((void**)&c.A.jd, (size_t)nnz * sizeof(short)));
+      EVSL_HIP(hipMemcpy(c.A.jd, jd.data(), (size_t)nnz * sizeof(short), hipMemcpyHostToDevice));
+    }
+  }
   size_t need = (size_t)evsl_dev::kRedBlocks * 64;
   for (int w = 0; w < evsl_dev::kNumWidths; ++w) {
     const int B = evsl_dev::width_of_slot(w);

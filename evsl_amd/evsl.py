@@ -41,7 +41,7 @@ EXPORTS = [
     "evsl_amd_set_refine_passes", "evsl_amd_set_refine_guard", "evsl_amd_get_stats", "evsl_amd_reset_stats",
     "evsl_amd_last_error", "evsl_amd_stats_size", "kpmdos_ex", "evsl_amd_bench_filter",
     "ChebLanNr_batch", "evsl_amd_read_coo_mm", "evsl_amd_bench_copy",
-    "evsl_amd_spmm_vec",
+    "evsl_amd_spmm_vec", "evsl_amd_spmm_index_bytes",
 ]
 
 
@@ -121,6 +121,7 @@ class EVSL:
             "evsl_amd_bench_filter": (I, [I, I, I, I, D, D, dp, dp]),
             "evsl_amd_bench_copy": (I, [C.c_size_t, I, dp]),
             "evsl_amd_spmm_vec": (I, [I, I, I]),
+            "evsl_amd_spmm_index_bytes": (I, []),
             "ChebLanNr_batch": (I, [I, dp, I, D, dp, pp, ip, ddp, ddp, ddp, VP]),
         }
         for name, (res, args) in sig.items():
@@ -356,3 +357,10 @@ class EVSL:
 
     def spmm_vec(self, B, prec, mode):
         return self.lib.evsl_amd_spmm_vec(B, prec, mode)
+
+    def spmm_kernel_name(self, B, prec, mode):
+        """Template name of the fused filter kernel for (B, prec, mode) on
+        the current matrix, as rocprofv3 reports it."""
+        vt = "float" if prec == PREC_MIXED and mode != 2 else "double"
+        i16 = "true" if self.lib.evsl_amd_spmm_index_bytes() == 2 else "false"
+        return f"k_cheb_spmm<{vt}, {B}, {self.spmm_vec(B, prec, mode)}, {mode}, true, {i16}>"

@@ -95,6 +95,10 @@ int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, doub
  * for (B, prec, mode): its template instance is
  * k_cheb_spmm<float|double, B, V, mode, true>; -1 for an unsupported B. */
 int evsl_amd_spmm_vec(int B, int prec, int mode);
+/* Bytes per nonzero of the column index the filter kernels read for the
+ * current matrix: 2 when every |ja - row| fits in 16 bits (the kernels then
+ * run as k_cheb_spmm<..., true, true>), 4 otherwise; 0 without a matrix. */
+int evsl_amd_spmm_index_bytes(void);
 
 /* Measurement hook: mean time of a device copy of `bytes` (16-byte
  * accesses) over reps launches -- the achievable-bandwidth reference

@@ -477,6 +477,7 @@ int ora_kpmdos(int Mdeg, int damping, int nvec, const double *intv, double *mu,
 /* yi[j] = integral from xi[0] to xi[j] of the KPM density, in closed form
  * (SRC/spslice.c:155-181) */
 void ora_intChx(int Mdeg, const double *mu, int npts, const double *xi, double *yi) {
+  if (npts <= 0) return;
   double *thetas = malloc(npts * sizeof(double));
   for (int j = 0; j < npts; j++) thetas[j] = acos(xi[j]);
   double theta0 = thetas[0];

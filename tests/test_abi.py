@@ -162,3 +162,27 @@ def test_read_coo_mm_matches_scipy(lib_ev):
     assert np.array_equal(A.ia, ref.indptr) and np.array_equal(A.ja, ref.indices)
     assert np.array_equal(A.a, ref.data)
     assert lib_ev.read_mm(path + ".missing")[0] < 0
+
+
+def test_fails_loudly_without_a_gpu(lib_ev):
+    """Without a usable HIP device every device entry point returns
+    EVSL_ERR_DEVICE (-101) with a message; nothing falls back to the CPU.
+    (On a machine with a GPU the context starts and the test is skipped.)"""
+    from evsl_amd.types import CsrMat
+    err = lib_ev.lib.EVSLStart()
+    if err == 0:
+        lib_ev.lib.EVSLFinish()
+        pytest.skip("a HIP device is present")
+    assert err == -101
+    assert "device" in lib_ev.last_error().lower()
+    A = M.laplacian(10, 10, 1)
+    assert lib_ev.set_matrix(A) == -101
+    intv = np.array([0.5, 1.0, 0.0, 8.0])
+    assert lib_ev.kpmdos(20, 1, 2, intv)[0] == -101
+    pol = lib_ev.default_pol()
+    assert lib_ev.find_pol(intv, pol) == 0  # host-only filter design still works
+    v = np.ones(A.n)
+    assert lib_ev.ChebLanNr(intv, 50, 1e-8, v, pol)[0] == -101
+    assert lib_ev.ChebLanTr(60, 4, intv, 100, 1e-8, v, pol)[0] == -101
+    assert lib_ev.LanTrbounds(20, 50, 1e-8, v, 1)[0] == -101
+    lib_ev.free_pol(pol)
