@@ -42,17 +42,20 @@ int pow2_at_least(int b) {
 // block widths the kCheb kernel is instantiated for
 int kernel_width(int b) { return b <= 8 ? b : pow2_at_least(b); }
 
-// Run the Chebyshev filter of per-column coefficient table mus ((maxdeg+1) x B,
-// row d holds degree d) over the row-major block X0 of B columns; result in Y.
-int block_filter(int B, const FilterOp& op, const double* mus_d, int maxdeg, const void* X0,
-                 void* const bufs[3], void* Y) {
+// Launches k = k0..k1 of the Chebyshev filter of per-column coefficient
+// table mus (row d holds degree d, B entries per row) over the row-major
+// block X0 of B columns, accumulating into Y.  The y flushes are scheduled
+// within the segment, so its last launch leaves no pending term; ring slot
+// (k-1)%3 of bufs receives T_k.
+int block_filter_seg(int B, const FilterOp& op, const double* mus_d, int k0, int k1,
+                     const void* X0, void* const bufs[3], void* Y) {
   Context& c = ctx();
   const size_t vs = op.f32 ? 4 : 8;
-  for (int k = 1; k <= maxdeg; ++k) {
+  for (int k = k0; k <= k1; ++k) {
     void* xkp1 = bufs[(k - 1) % 3];
     const void* xk = (k == 1) ? X0 : bufs[(k - 2) % 3];
     const void* xkm1 = (k == 1) ? nullptr : (k == 2 ? X0 : bufs[(k - 3) % 3]);
-    const int yf = evsl_dev::y_flush(k, maxdeg);
+    const int yf = evsl_dev::y_flush(k - k0 + 1, k1 - k0 + 1);
     EVSL_HIP(evsl_dev::cheb_spmm(op.A, B, op.f32, evsl_dev::kCheb, xk, xkm1, xkp1, Y, nullptr,
                                  mus_d + (size_t)k * B, mus_d, k == 1 ? op.t1 : op.t2, op.cc,
                                  k == 1, yf, nullptr, c.stream));
@@ -60,14 +63,20 @@ int block_filter(int B, const FilterOp& op, const double* mus_d, int maxdeg, con
     c.astats.filter_bytes += (double)c.A.nnz * (vs + c.A.idx_bytes()) + 4.0 * (c.n + 1) +
                              (double)streams * B * c.n * vs;
   }
-  c.astats.n_filter_steps += maxdeg;
-  c.astats.n_filter_cols += (size_t)maxdeg * B;
+  c.astats.n_filter_steps += k1 - k0 + 1;
+  c.astats.n_filter_cols += (size_t)(k1 - k0 + 1) * B;
   return 0;
+}
+
+// The whole series k = 1..maxdeg at one width.
+int block_filter(int B, const FilterOp& op, const double* mus_d, int maxdeg, const void* X0,
+                 void* const bufs[3], void* Y) {
+  return block_filter_seg(B, op, mus_d, 1, maxdeg, X0, bufs, Y);
 }
 
 // Shared work buffers of the blocked filter (sized for Bmax columns).
 struct BlockWork {
-  DevBuf X0, bufs, Y, zero, mus, colptr_in, colptr_out;
+  DevBuf X0, bufs, Y, Y2, zero, mus, colptr_in, colptr_out;
   int Bcap = 0;
   bool f32 = false;
   int ensure(int n, int B, bool fp32, int maxdeg) {
@@ -76,6 +85,7 @@ struct BlockWork {
     EVSL_HIP(X0.alloc(nb));
     EVSL_HIP(bufs.alloc(3 * nb));
     EVSL_HIP(Y.alloc(nb));
+    EVSL_HIP(Y2.alloc(nb));
     EVSL_HIP(zero.alloc((size_t)n * sizeof(double)));
     EVSL_HIP(mus.alloc((size_t)(maxdeg + 1) * B * sizeof(double)));
     EVSL_HIP(colptr_in.alloc(B * sizeof(double*)));
@@ -95,6 +105,76 @@ struct BlockWork {
 int upload_ptrs(DevBuf& dst, const void* const* p, int cnt) {
   EVSL_HIP(dst.alloc((size_t)cnt * sizeof(void*)));
   EVSL_HIP(hipMemcpy(dst.p, p, (size_t)cnt * sizeof(void*), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Columns of different degrees (b <= 8, every degree >= 3): ordered by
+// decreasing degree, the series runs at width b up to the smallest degree,
+// whose columns are then written out and dropped; the live recurrence
+// vectors T_k, T_{k-1} and the partial sums Y are narrowed to the remaining
+// columns, and so on.  Each column sees the same arithmetic as in one
+// full-width run (per-column sums do not depend on the block width).
+int filter_columns_phased(int b, const double* const* xin, double* const* yout,
+                          const polparams* const* pols, bool f32, BlockWork& W) {
+  Context& c = ctx();
+  const int n = c.n;
+  std::vector<int> ord(b);
+  for (int i = 0; i < b; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(),
+                   [&](int x, int y) { return pols[x]->deg > pols[y]->deg; });
+  const int md = pols[ord[0]]->deg;
+  if (int e = W.ensure(n, b, f32, md)) return e;
+  std::vector<const double*> in(b);
+  std::vector<double*> out(b);
+  for (int i = 0; i < b; ++i) {
+    in[i] = xin[ord[i]];
+    out[i] = yout[ord[i]];
+  }
+  if (int e = upload_ptrs(W.colptr_in, (const void* const*)in.data(), b)) return e;
+  if (int e = upload_ptrs(W.colptr_out, (const void* const*)out.data(), b)) return e;
+  EVSL_HIP(evsl_dev::pack_cols(n, b, W.colptr_in.as<const double*>(), W.X0.p, f32, c.stream));
+  FilterOp op;
+  if (int e = make_filter_op(pols[ord[0]]->cc, pols[ord[0]]->dd, f32, &op)) return e;
+  void* slot[3] = {W.buf(0, n, b), W.buf(1, n, b), W.buf(2, n, b)};
+  void* Y = W.Y.p;
+  void* Yspare = W.Y2.p;
+  double* const* outp = W.colptr_out.as<double* const>();
+  const size_t vs = f32 ? 4 : 8;
+  int active = b, k0 = 1;
+  std::vector<double> mus;
+  while (active > 0) {
+    const int kend = pols[ord[active - 1]]->deg;
+    // coefficient rows 0..kend for the active columns
+    mus.assign((size_t)(kend + 1) * active, 0.0);
+    for (int i = 0; i < active; ++i)
+      for (int d = 0; d <= kend; ++d) mus[(size_t)d * active + i] = pols[ord[i]]->mu[d];
+    EVSL_HIP(hipMemcpy(W.mus.p, mus.data(), mus.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (int e = block_filter_seg(active, op, W.mus.as<double>(), k0, kend, W.X0.p, slot, Y))
+      return e;
+    int keep = active;
+    while (keep > 0 && pols[ord[keep - 1]]->deg == kend) --keep;
+    EVSL_HIP(evsl_dev::unpack_range(n, active, keep, active, Y, f32, outp + keep, c.stream));
+    if (keep > 0) {
+      // T_kend sits in slot (kend-1)%3 and T_{kend-1} in slot (kend-2)%3
+      // (kend >= 3); narrow them into the free slots and swap the pointers
+      const int sa = (kend - 1) % 3, sb = (kend - 2) % 3, sf = kend % 3;
+      void* pa = slot[sa];
+      void* pb = slot[sb];
+      void* pf = slot[sf];
+      EVSL_HIP(evsl_dev::narrow_cols(n, active, keep, pa, pf, f32, c.stream));
+      EVSL_HIP(evsl_dev::narrow_cols(n, active, keep, pb, pa, f32, c.stream));
+      slot[sa] = pf;
+      slot[sb] = pa;
+      slot[sf] = pb;
+      EVSL_HIP(evsl_dev::narrow_cols(n, active, keep, Y, Yspare, f32, c.stream));
+      std::swap(Y, Yspare);
+      c.astats.filter_bytes += 3.0 * (double)n * (active + keep) * vs;
+    }
+    active = keep;
+    k0 = kend + 1;
+  }
+  // the pointer tables are reused by the next chunk: wait for this one
+  EVSL_HIP(hipStreamSynchronize(c.stream));
   return 0;
 }
 
@@ -133,6 +213,14 @@ int filter_columns(int ncols, const double* const* xin, double* const* yout,
     if (md == 0) {  // degree-0 filters: y = mu0 x
       for (int cc = 0; cc < b; ++cc)
         EVSL_HIP(evsl_dev::scal_copy(n, pols[j0 + cc]->mu[0], in[cc], out[cc], c.stream));
+      continue;
+    }
+    // mixed degrees in a block of at most 8 columns: run the series in
+    // segments, dropping the columns whose degree is reached
+    int dmin = md;
+    for (int cc = 0; cc < b; ++cc) dmin = std::min(dmin, pols[j0 + cc]->deg);
+    if (B == b && b > 1 && dmin < md && dmin >= 3) {
+      if (int e = filter_columns_phased(b, xin + j0, yout + j0, pols + j0, f32, W)) return e;
       continue;
     }
     EVSL_HIP(hipMemcpy(W.mus.p, mus.data(), mus.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -346,7 +434,7 @@ int refine_extract(int n, int kdim, const double* dT, const double* eT, const do
   if (kdim > 1) std::memcpy(e.data(), eT, (kdim - 1) * sizeof(double));
   RotationLog rec;
   if (int err = tridiag_ql(kdim, eval.data(), e.data(), &rec)) return err;
-  // guard threshold (default half of bar): keep Ritz values of p(A) so that
+  // guard threshold (default 0.6 bar): keep Ritz values of p(A) so that
   // eigenvectors just outside [a, b] are in the Rayleigh-Ritz space
   const double thr = mixed ? c.refine_guard * bar : bar;
   std::vector<int> sel;

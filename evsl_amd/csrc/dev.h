@@ -185,6 +185,13 @@ hipError_t rm_random(size_t n, int B, int kind, unsigned long long seed, int col
 // scale[c] if scale != nullptr).
 hipError_t pack_cols(size_t n, int B, const double* const* cols, void* X, bool fp32,
                      hipStream_t st);
+// dst (n x B2) = the first B2 columns of the interleaved n x B block src
+// (separate buffers)
+hipError_t narrow_cols(size_t n, int B, int B2, const void* src, void* dst, bool fp32,
+                       hipStream_t st);
+// cols[c - c0] (FP64 device vectors) = column c of the interleaved block, c0 <= c < c1
+hipError_t unpack_range(size_t n, int B, int c0, int c1, const void* X, bool fp32,
+                        double* const* cols, hipStream_t st);
 hipError_t unpack_cols(size_t n, int B, const void* X, bool fp32, double* const* cols,
                        hipStream_t st);
 // row-major block <-> column-major panel (ld) of B columns starting at col0

@@ -82,7 +82,7 @@ struct Context {
   hipStream_t stream = nullptr;  // the legacy null stream (serialises with user code)
   int filter_prec = EVSL_PREC_FP64;
   int refine_passes = 1;
-  double refine_guard = 0.5;
+  double refine_guard = 0.6;
   // scratch for reductions
   DevBuf red_part, red_scalars;
   // pinned host scalars for device->host reads

@@ -797,6 +797,32 @@ __global__ void k_unpack_cols(size_t n, int B, const T* __restrict__ X,
   }
 }
 
+// dst (n x B2, interleaved) = the first B2 columns of src (n x B, interleaved)
+template <typename T>
+__global__ void k_narrow_cols(size_t n, int B, int B2, const T* __restrict__ src,
+                              T* __restrict__ dst) {
+  const size_t tot = n * (size_t)B2;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = i / B2;
+    dst[i] = src[r * B + (i - r * B2)];
+  }
+}
+
+// cols[c - c0][r] = X[r*B + c] for the columns c0 <= c < c1 of an interleaved block
+template <typename T>
+__global__ void k_unpack_range(size_t n, int B, int c0, int c1, const T* __restrict__ X,
+                               double* const* __restrict__ cols) {
+  const int w = c1 - c0;
+  const size_t tot = n * (size_t)w;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = i / w;
+    const int c = (int)(i - r * w);
+    cols[c][r] = (double)X[r * B + c0 + c];
+  }
+}
+
 template <bool TO_CM>
 __global__ void k_rm_cm(size_t n, int B, const double* __restrict__ src,
                         double* __restrict__ dst, size_t ld) {
@@ -1309,6 +1335,32 @@ hipError_t unpack_cols(size_t n, int B, const void* X, bool fp32, double* const*
                        cols);
   else
     hipLaunchKernelGGL(k_unpack_cols<double>, dim3(g), dim3(kWG), 0, st, n, B,
+                       (const double*)X, cols);
+  return hipGetLastError();
+}
+
+hipError_t narrow_cols(size_t n, int B, int B2, const void* src, void* dst, bool fp32,
+                       hipStream_t st) {
+  if (B2 <= 0 || B2 > B) return hipErrorInvalidValue;
+  const unsigned grid = grid_for(n * (size_t)B2, kWG, 8192);
+  if (fp32)
+    hipLaunchKernelGGL(k_narrow_cols<float>, dim3(grid), dim3(kWG), 0, st, n, B, B2,
+                       (const float*)src, (float*)dst);
+  else
+    hipLaunchKernelGGL(k_narrow_cols<double>, dim3(grid), dim3(kWG), 0, st, n, B, B2,
+                       (const double*)src, (double*)dst);
+  return hipGetLastError();
+}
+
+hipError_t unpack_range(size_t n, int B, int c0, int c1, const void* X, bool fp32,
+                        double* const* cols, hipStream_t st) {
+  if (c0 < 0 || c1 > B || c0 >= c1) return hipErrorInvalidValue;
+  const unsigned grid = grid_for(n * (size_t)(c1 - c0), kWG, 8192);
+  if (fp32)
+    hipLaunchKernelGGL(k_unpack_range<float>, dim3(grid), dim3(kWG), 0, st, n, B, c0, c1,
+                       (const float*)X, cols);
+  else
+    hipLaunchKernelGGL(k_unpack_range<double>, dim3(grid), dim3(kWG), 0, st, n, B, c0, c1,
                        (const double*)X, cols);
   return hipGetLastError();
 }

@@ -120,7 +120,7 @@ int tridiag_eig_host(int n, const double* diag, const double* sdiag, double* eva
 int sym_eig_host(int n, const double* A, int lda, double* Q, int ldq, double* lam) {
   // Householder reduction W = H^T A H (H accumulated in Z), then QL on the
   // tridiagonal with rotations replayed on Z.
-  std::vector<double> W((size_t)n * n), Z((size_t)n * n, 0.0), v(n), p(n);
+  std::vector<double> W((size_t)n * n), Z((size_t)n * n, 0.0), v(n), p(n), zv(n);
   for (int j = 0; j < n; ++j)
     for (int i = 0; i < n; ++i) W[i + (size_t)j * n] = A[i + (size_t)j * lda];
   for (int i = 0; i < n; ++i) Z[i + (size_t)i * n] = 1.0;
@@ -139,28 +139,40 @@ int sym_eig_host(int n, const double* A, int lda, double* Q, int ldq, double* la
     vn = std::sqrt(vn);
     if (vn == 0.0) continue;
     for (int i = 0; i < len; ++i) v[i] /= vn;
-    double K = 0.0;
-    for (int i = 0; i < len; ++i) {
-      double s = 0.0;
-      for (int j = 0; j < len; ++j) s += W[(k + 1 + i) + (size_t)(k + 1 + j) * n] * v[j];
-      p[i] = s;
-      K += v[i] * s;
+    // p = W v by columns (contiguous), K = v.p
+    for (int i = 0; i < len; ++i) p[i] = 0.0;
+    for (int j = 0; j < len; ++j) {
+      const double* wc = &W[(k + 1) + (size_t)(k + 1 + j) * n];
+      const double vj = v[j];
+      for (int i = 0; i < len; ++i) p[i] += wc[i] * vj;
     }
+    double K = 0.0;
+    for (int i = 0; i < len; ++i) K += v[i] * p[i];
     for (int i = 0; i < len; ++i) p[i] = 2.0 * (p[i] - K * v[i]);
-    for (int j = 0; j < len; ++j)
-      for (int i = 0; i < len; ++i)
-        W[(k + 1 + i) + (size_t)(k + 1 + j) * n] -= v[i] * p[j] + p[i] * v[j];
+#pragma omp parallel for schedule(static) if (len > 192)
+    for (int j = 0; j < len; ++j) {
+      double* wc = &W[(k + 1) + (size_t)(k + 1 + j) * n];
+      const double vj = v[j], pj = p[j];
+      for (int i = 0; i < len; ++i) wc[i] -= v[i] * pj + p[i] * vj;
+    }
     W[(k + 1) + (size_t)k * n] = alpha;
     W[k + (size_t)(k + 1) * n] = alpha;
     for (int i = 1; i < len; ++i) {
       W[(k + 1 + i) + (size_t)k * n] = 0.0;
       W[k + (size_t)(k + 1 + i) * n] = 0.0;
     }
-    for (int r = 0; r < n; ++r) {
-      double s = 0.0;
-      for (int i = 0; i < len; ++i) s += Z[r + (size_t)(k + 1 + i) * n] * v[i];
-      s *= 2.0;
-      for (int i = 0; i < len; ++i) Z[r + (size_t)(k + 1 + i) * n] -= s * v[i];
+    // Z(:, k+1:) -= 2 (Z(:, k+1:) v) v^T, column by column
+    for (int r = 0; r < n; ++r) zv[r] = 0.0;
+    for (int i = 0; i < len; ++i) {
+      const double* zc = &Z[(size_t)(k + 1 + i) * n];
+      const double vi = v[i];
+      for (int r = 0; r < n; ++r) zv[r] += zc[r] * vi;
+    }
+#pragma omp parallel for schedule(static) if (len > 192)
+    for (int i = 0; i < len; ++i) {
+      double* zc = &Z[(size_t)(k + 1 + i) * n];
+      const double vi2 = 2.0 * v[i];
+      for (int r = 0; r < n; ++r) zc[r] -= zv[r] * vi2;
     }
   }
   std::vector<double> e(n + 1, 0.0);
@@ -173,11 +185,14 @@ int sym_eig_host(int n, const double* A, int lda, double* Q, int ldq, double* la
   std::vector<double> R((size_t)n * n);
   for (int r = 0; r < n; ++r)
     for (int c = 0; c < n; ++c) R[(size_t)r * n + c] = Z[r + (size_t)c * n];
-  for (size_t sw = 0; sw < rec.top.size(); ++sw) {
-    const int ihi = rec.top[sw], cnt = rec.cnt[sw];
-    const double* rot = rec.cs.data() + 2 * rec.off[sw];
-    for (int r = 0; r < n; ++r) {
-      double* z = R.data() + (size_t)r * n;
+  // rows are independent: each applies every sweep in order
+  const size_t nsw = rec.top.size();
+#pragma omp parallel for schedule(static) if (n > 128)
+  for (int r = 0; r < n; ++r) {
+    double* z = R.data() + (size_t)r * n;
+    for (size_t sw = 0; sw < nsw; ++sw) {
+      const int ihi = rec.top[sw], cnt = rec.cnt[sw];
+      const double* rot = rec.cs.data() + 2 * rec.off[sw];
       double carry = z[ihi + 1];
       for (int j = 0; j < cnt; ++j) {
         const int i = ihi - j;

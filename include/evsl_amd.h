@@ -48,7 +48,7 @@ int evsl_amd_get_filter_precision(void);
 int evsl_amd_set_refine_passes(int passes);
 /* guard threshold of the refinement: Ritz values of p(A) down to
  * guard * bar enter the FP64 Rayleigh-Ritz space (0 < guard <= 1,
- * default 0.5) */
+ * default 0.6) */
 int evsl_amd_set_refine_guard(double guard);
 
 int evsl_amd_get_stats(evslAmdStats *s);
