@@ -186,6 +186,12 @@ int filter_columns(int ncols, const double* const* xin, double* const* yout,
                    const polparams* const* pols, bool f32, BlockWork& W) {
   Context& c = ctx();
   const int n = c.n;
+  if (c.op != 1) {  // user operator: one column at a time through its MVFunc, FP64
+    ChebWork cw;
+    for (int j = 0; j < ncols; ++j)
+      if (int e = cheb_av(pols[j], xin[j], yout[j], cw, EVSL_PREC_FP64)) return e;
+    return 0;
+  }
   int maxdeg = 0;
   for (int j = 0; j < ncols; ++j) maxdeg = std::max(maxdeg, pols[j]->deg);
   DevBuf junk;

@@ -396,3 +396,73 @@ def test_chebsi_parity(ev, oracle, libc, case):
     np.testing.assert_allclose(np.sort(lg), np.sort(lo), rtol=1e-10, atol=1e-13)
     np.testing.assert_allclose(np.sort(lg), exact, rtol=1e-10, atol=1e-12)
     check_pairs(A, lg, Yg, rg, intv, 1e-8)
+
+
+def test_solvers_through_user_matvec(ev, oracle, libc):
+    """ChebLanTr, ChebSI, the batched ChebLanNr and LanTrbounds through a
+    user MVFunc (SetAMatvec, device vectors in the callback) give the
+    closed-form spectrum, and the same results as through SetAMatrix."""
+    A = M.laplacian(24, 24, 1)
+    S = A.to_scipy()
+    n = A.n
+    intv = np.array([0.5, 0.9, 0.0, 8.0])
+    exact = M.laplacian_eigs(24, 24, 1, 0.5, 0.9)
+    libc.srand(31)
+    v = oracle.rand_double(n)
+    libc.srand(32)
+    V0 = oracle.rand_double(n * (len(exact) + 4)).reshape(len(exact) + 4, n).T
+    buf = np.empty(n)
+
+    def mv(x, y, _):
+        ev.lib.evsl_memcpy_device_to_host(buf.ctypes.data, C.cast(x, C.c_void_p), 8 * n)
+        out = np.ascontiguousarray(S @ buf)
+        ev.lib.evsl_memcpy_host_to_device(C.cast(y, C.c_void_p), out.ctypes.data, 8 * n)
+
+    results = []
+    for mode in ("csr", "matvec"):
+        if mode == "csr":
+            assert ev.set_matrix(A) == 0
+        else:
+            assert ev.set_matvec(n, mv) == 0
+        pol = slice_pol(ev, intv)
+        etr, ltr, _, _ = ev.ChebLanTr(300, len(exact) + 2, intv, 900, 1e-8, v, pol)
+        psi = ev.default_pol()
+        psi.damping, psi.thresh_int = 0, 0.5
+        assert ev.find_pol(intv, psi) == 0
+        esi, lsi, _, _ = ev.ChebSI(len(exact) + 4, intv, 1000, 1e-8, V0, psi)
+        eb, lo, hi = ev.LanTrbounds(50, 200, 1e-8, v, 1)
+        ebt, outs = ev.ChebLanNr_batch([intv, intv], 400, 1e-8, v, [pol, pol])
+        ev.free_pol(pol)
+        ev.free_pol(psi)
+        assert etr == 0 and esi == 0 and eb == 0 and ebt == 0, ev.last_error()
+        for lam in (ltr, lsi, outs[0][0], outs[1][0]):
+            assert len(lam) == len(exact)
+            np.testing.assert_allclose(np.sort(lam), exact, rtol=1e-10)
+        results.append((np.sort(ltr), np.sort(lsi), lo, hi))
+    (t1, s1, lo1, hi1), (t2, s2, lo2, hi2) = results
+    np.testing.assert_allclose(t1, t2, rtol=1e-10)
+    np.testing.assert_allclose(s1, s2, rtol=1e-10)
+    assert lo1 == pytest.approx(lo2, rel=1e-9, abs=1e-12) and hi1 == pytest.approx(hi2, rel=1e-9)
+
+
+def test_solver_error_returns(ev):
+    """Reference error conventions for the added solvers: an empty interval
+    returns 0 with no pairs (ChebLanTr, ChebSI), |gam| > 1 returns -1, and a
+    Krylov dimension too small for the restart set returns EVSL_ERR_ARG
+    (the reference exits)."""
+    A = M.laplacian(20, 20, 1)
+    assert ev.set_matrix(A) == 0
+    v = np.random.default_rng(3).uniform(-1, 1, A.n)
+    pol = slice_pol(ev, np.array([0.5, 0.9, 0.0, 8.0]))
+    err, lam, _, _ = ev.ChebLanTr(100, 5, np.array([0.9, 0.5, 0.0, 8.0]), 300, 1e-8, v, pol)
+    assert err == 0 and len(lam) == 0
+    err, lam, _, _ = ev.ChebSI(5, np.array([0.9, 0.5, 0.0, 8.0]), 100, 1e-8,
+                               np.ones((A.n, 5)), pol)
+    assert err == 0 and len(lam) == 0
+    gam = pol.gam
+    pol.gam = 2.0
+    assert ev.ChebLanTr(100, 5, np.array([0.5, 0.9, 0.0, 8.0]), 300, 1e-8, v, pol)[0] == -1
+    assert ev.ChebSI(5, np.array([0.5, 0.9, 0.0, 8.0]), 100, 1e-8, np.ones((A.n, 5)), pol)[0] == -1
+    pol.gam = gam
+    assert ev.LanTrbounds(4, 50, 1e-8, v, 1)[0] == -104
+    ev.free_pol(pol)
