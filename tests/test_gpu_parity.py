@@ -466,3 +466,21 @@ def test_solver_error_returns(ev):
     pol.gam = gam
     assert ev.LanTrbounds(4, 50, 1e-8, v, 1)[0] == -104
     ev.free_pol(pol)
+
+
+def test_landos_matches_exact_dos_on_device(ev, libc):
+    """Device LanDos on the reference LanDos test problem (3-D 20^3 Laplacian,
+    msteps 40, npts 200, nvec 100) against the exact smoothed DOS of the
+    closed-form spectrum (SRC/exDOS.c)."""
+    from test_oracle_kpm import exact_smoothed_dos
+    A = M.laplacian(20, 20, 20)
+    assert ev.set_matrix(A) == 0
+    intv = np.array([0.0, 12.0, 0.0, 12.0])
+    libc.srand(2)
+    err, x, y, neig = ev.LanDos(100, 40, 200, intv)
+    assert err == 0
+    lam = M.laplacian_eigs(20, 20, 20, -1.0, 13.0)
+    yex = exact_smoothed_dos(lam, x, intv, 40)
+    assert np.abs(y - yex).max() < 0.04 * yex.max()
+    assert np.abs(y - yex).sum() < 0.02 * yex.sum()
+    assert abs(neig - len(lam)) < 0.01 * len(lam)

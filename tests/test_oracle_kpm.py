@@ -155,3 +155,41 @@ def test_landos_count(oracle, libc):
     np.testing.assert_allclose(x, np.linspace(0.5, 2.5, 200), rtol=0, atol=1e-15)
     assert np.all(y >= 0)
     assert abs(neig - exact) < 0.15 * exact
+
+
+def exact_smoothed_dos(vals, x, intv, msteps):
+    """The reference's exact DOS (SRC/exDOS.c, used by TESTS/Landos/LanDos.c):
+    Gaussians of width sigma = H / sqrt(8 ln 1.25), H = (lmax - lmin)/(M - 1),
+    M = min(n, msteps, 30), summed at the grid points and normalised to unit
+    mass (1 / (n sqrt(2 pi sigma^2)))."""
+    n = len(vals)
+    m = min(n, msteps, 30)
+    h = (intv[3] - intv[2]) / (m - 1)
+    sigma = h / np.sqrt(8 * np.log(1.25))
+    s2 = 2 * sigma * sigma
+    width = sigma * np.sqrt(-2 * np.log(1e-8))
+    y = np.zeros_like(x)
+    for t in vals:
+        if t < intv[0] or t > intv[1]:
+            continue
+        sel = np.abs(x - t) <= width
+        y[sel] += np.exp(-(x[sel] - t) ** 2 / s2)
+    return y / (n * np.sqrt(s2 * np.pi))
+
+
+def test_landos_matches_exact_dos(oracle, libc):
+    """The reference LanDos test problem (TESTS/Landos/LanDos.c: 3-D 20^3
+    Laplacian, msteps 40, npts 200, nvec 100, the whole spectrum [0, 12]):
+    the Lanczos DOS agrees with the exact smoothed DOS of the closed-form
+    eigenvalues to a few percent of its peak, and the count to 1%."""
+    A = M.laplacian(20, 20, 20)
+    oracle.set_csr(A)
+    intv = np.array([0.0, 12.0, 0.0, 12.0])
+    libc.srand(2)
+    err, x, y, neig = oracle.LanDos(100, 40, 200, intv)
+    assert err == 0
+    ev = M.laplacian_eigs(20, 20, 20, -1.0, 13.0)
+    yex = exact_smoothed_dos(ev, x, intv, 40)
+    assert np.abs(y - yex).max() < 0.04 * yex.max()
+    assert np.abs(y - yex).sum() < 0.02 * yex.sum()
+    assert abs(neig - len(ev)) < 0.01 * len(ev)
