@@ -82,6 +82,8 @@ hipError_t stream_copy(size_t bytes, const void* src, void* dst, hipStream_t st)
 // columns per lane of the cheb_spmm kernel used for this width, precision
 // and mode (-1 for an unsupported width)
 int cheb_spmm_vec(int B, bool fp32, Mode mode);
+// rows per lane group of that kernel (row blocks of R x the lane groups)
+int cheb_spmm_rows(int B, bool fp32, Mode mode);
 
 // number of row blocks (KPM partials per column) one cheb_spmm launch of
 // this width, precision and mode uses; -1 for an unsupported width
