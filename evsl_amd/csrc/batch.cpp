@@ -222,10 +222,12 @@ int filter_columns(int ncols, const double* const* xin, double* const* yout,
       continue;
     }
     // mixed degrees in a block of at most 8 columns: run the series in
-    // segments, dropping the columns whose degree is reached
+    // segments, dropping the columns whose degree is reached (every width
+    // 1..8 has a kernel; wider blocks run all columns to the largest degree,
+    // the others with zero coefficients)
     int dmin = md;
     for (int cc = 0; cc < b; ++cc) dmin = std::min(dmin, pols[j0 + cc]->deg);
-    if (B == b && b > 1 && dmin < md && dmin >= 3) {
+    if (B == b && b > 1 && b <= 8 && dmin < md && dmin >= 3) {
       if (int e = filter_columns_phased(b, xin + j0, yout + j0, pols + j0, f32, W)) return e;
       continue;
     }
@@ -1112,6 +1114,59 @@ int ChebSI(int nev, double* intv, int maxit, double tol, double* vinit, polparam
     std::fprintf(fstats, "------This slice consumed: \nMatvecs :        %ld\ntotal  time :    %.2f\n",
                  nmv, tall);
   evslstat.t_iter = tall;
+  return 0;
+}
+
+
+// Y(:, j) = p_j(A) X(:, j) for the ncols columns of the n x ncols
+// column-major X (host or device; Y host or device likewise), with the
+// blocked filter: at most 32 columns per series, columns of different degrees
+// narrowed as their degrees are reached.  The block form of the reference's
+// ChebAv (SRC/chebpoly.c:519-598), one p_j per column; every pols[j] must
+// come from find_pol on the same outer bounds (same cc, dd).
+int evsl_amd_cheb_av_cols(int ncols, const polparams* pols, const double* X, double* Y,
+                          int prec) {
+  if (int e = ensure_started()) return e;
+  Context& c = ctx();
+  if (c.op == 0) return EVSL_ERR_NOMAT;
+  if (ncols <= 0 || !pols || !X || !Y || (prec != EVSL_PREC_FP64 && prec != EVSL_PREC_MIXED)) {
+    set_error_msg("evsl_amd_cheb_av_cols: invalid arguments");
+    return EVSL_ERR_ARG;
+  }
+  for (int j = 0; j < ncols; ++j)
+    if (!pols[j].mu || pols[j].deg < 0 || pols[j].cc != pols[0].cc ||
+        pols[j].dd != pols[0].dd) {
+      set_error_msg("evsl_amd_cheb_av_cols: every filter needs coefficients and the same cc, dd");
+      return EVSL_ERR_ARG;
+    }
+  const size_t nl = c.n, tot = nl * ncols;
+  DevBuf xt, yt;
+  const double* xd;
+  if (int e = as_device_vector(X, tot, xt, &xd)) return e;
+  double* yd = Y;
+  hipPointerAttribute_t at;
+  const bool ydev = hipPointerGetAttributes(&at, Y) == hipSuccess &&
+                    (at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged);
+  if (!ydev) {
+    (void)hipGetLastError();  // clear the error of an unregistered host pointer
+    EVSL_HIP(yt.alloc(tot * sizeof(double)));
+    yd = yt.as<double>();
+  }
+  std::vector<const double*> in(ncols);
+  std::vector<double*> out(ncols);
+  std::vector<const polparams*> pp(ncols);
+  for (int j = 0; j < ncols; ++j) {
+    in[j] = xd + (size_t)j * nl;
+    out[j] = yd + (size_t)j * nl;
+    pp[j] = pols + j;
+  }
+  BlockWork bw;
+  if (int e = filter_columns(ncols, in.data(), out.data(), pp.data(),
+                             prec == EVSL_PREC_MIXED, bw))
+    return e;
+  if (!ydev)
+    EVSL_HIP(hipMemcpyAsync(Y, yd, tot * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  EVSL_HIP(hipStreamSynchronize(c.stream));
   return 0;
 }
 

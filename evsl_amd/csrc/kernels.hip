@@ -956,38 +956,33 @@ hipError_t launch_variant(const Csr& A, const void* xk, const void* xkm1, void* 
   return hipGetLastError();
 }
 
-// Columns per lane for each (value size, block width, mode), chosen by a
-// sweep of every (V, LDS) layout on the 7-point 100^3 Laplacian on gfx950:
-// LDS staging won for every width, and blocks of 64 or more rows per 256
-// lanes did best (one column per lane gives only 256/B rows, so from B = 6
-// on, lanes take 2 or 4 adjacent columns).
-constexpr int spmm_vec(int vsize, int B, int mode) {
-  if (B == 2 || B == 6) return 2;
-  if (B == 4) return (vsize == 4 && mode == kKpm) ? 2 : 1;
-  if (B == 8) return (vsize == 8 && mode != kRes) ? 4 : 2;
-  if (B == 16) return (vsize == 4) == (mode != kKpm) ? 4 : 2;
-  if (B == 32) return (vsize == 8 && mode == kRes) ? 2 : 4;
-  return 1;  // B = 1, 3, 5, 7
-}
+// Layout of the fused filter kernel for each (value size, block width,
+// mode), as 10*V + R: V columns per lane and R rows per lane group.  With
+// V > 1 a lane takes V adjacent columns, so a block of 256 lanes covers more
+// than 256/B rows; with R > 1 a block covers R times the rows its lane groups
+// can hold at once and each group walks R of them, amortising the block's
+// table loads, staging and barrier (the row-block table is then the one built
+// for (B/V)/R lanes per row).  Chosen by a sweep of every valid (V, R) on the
+// 7-point 100^3 Laplacian on gfx950 (LDS staging won for every width); odd
+// widths only admit V = R = 1.  Time saved by the chosen R over the best
+// R = 1 layout: FP32 ChebAv 7% at 4 columns, 9% at 8, 15% at 16, 19% at 32;
+// FP64 ChebAv 2% at 4, 7% at 8, 11% at 16 and 32; the FP32 residual step
+// 7-8% at 2, 4, 16 and 32.  The FP32 KPM and FP64 residual rows (neither is
+// on a solver's hot path) keep the layouts of an earlier V-only sweep.
+//                                    B =  2   4   6   8  16  32
+constexpr int kSpmmLayout[2][3][6] = {{{21, 22, 21, 22, 44, 44},    // FP32 ChebAv
+                                       {21, 21, 21, 21, 21, 41},    // FP32 KPM
+                                       {12, 14, 21, 22, 44, 44}},   // FP32 residual
+                                      {{21, 12, 21, 22, 24, 24},    // FP64 ChebAv
+                                       {11, 12, 21, 41, 41, 41},    // FP64 KPM
+                                       {21, 11, 21, 22, 21, 22}}};  // FP64 residual
 
-// Rows per lane group: with R > 1 a block covers R times the rows its lane
-// groups can hold at once and each group walks R of them, amortising the
-// block's table loads, staging and barrier (the row-block table is then the
-// one built for (B/V)/R lanes per row).  Measured on the same sweep: R = 2
-// lifts the FP32 8-column step by 10% and the 32-column refinement steps by
-// 6-12%, R = 4 the FP32 32-column ChebAv step by 23%; one-column kernels
-// lose with R > 1.
-constexpr int spmm_rows(int vsize, int B, int mode) {
-  if (vsize == 4) {
-    if (B == 8 && (mode == kCheb || mode == kRes)) return 2;
-    if (B == 32 && mode == kCheb) return 4;
-    if (B == 32 && mode == kRes) return 2;
-  } else {
-    if (B == 32 && (mode == kCheb || mode == kRes)) return 2;
-    if (B == 8 && mode == kRes) return 2;
-  }
-  return 1;
+constexpr int spmm_layout(int vsize, int B, int mode) {
+  const int w = B == 2 ? 0 : B == 4 ? 1 : B == 6 ? 2 : B == 8 ? 3 : B == 16 ? 4 : B == 32 ? 5 : -1;
+  return w < 0 ? 11 : kSpmmLayout[vsize == 8][mode - kCheb][w];
 }
+constexpr int spmm_vec(int vsize, int B, int mode) { return spmm_layout(vsize, B, mode) / 10; }
+constexpr int spmm_rows(int vsize, int B, int mode) { return spmm_layout(vsize, B, mode) % 10; }
 
 template <typename VT, int B, int MODE>
 struct SpmmPolicy {

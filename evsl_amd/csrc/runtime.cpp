@@ -104,8 +104,11 @@ std::vector<int> make_row_blocks(const int* ia, int n, int rows_cap) {
 static Context g_ctx;
 Context& ctx() { return g_ctx; }
 
+// Called first by every device entry point: the error message then belongs
+// to the first failure of the current call.
 int ensure_started() {
   Context& c = g_ctx;
+  g_err.clear();
   if (c.started) return 0;
   return EVSLStart();
 }

@@ -41,7 +41,7 @@ EXPORTS = [
     "evsl_amd_set_refine_passes", "evsl_amd_set_refine_guard", "evsl_amd_get_stats", "evsl_amd_reset_stats",
     "evsl_amd_last_error", "evsl_amd_stats_size", "kpmdos_ex", "evsl_amd_bench_filter",
     "ChebLanNr_batch", "evsl_amd_read_coo_mm", "evsl_amd_bench_copy",
-    "evsl_amd_spmm_vec", "evsl_amd_spmm_index_bytes",
+    "evsl_amd_spmm_vec", "evsl_amd_spmm_index_bytes", "evsl_amd_cheb_av_cols",
 ]
 
 
@@ -122,6 +122,7 @@ class EVSL:
             "evsl_amd_bench_copy": (I, [C.c_size_t, I, dp]),
             "evsl_amd_spmm_vec": (I, [I, I, I]),
             "evsl_amd_spmm_index_bytes": (I, []),
+            "evsl_amd_cheb_av_cols": (I, [I, pp, dp, dp, I]),
             "ChebLanNr_batch": (I, [I, dp, I, D, dp, pp, ip, ddp, ddp, ddp, VP]),
         }
         for name, (res, args) in sig.items():
@@ -272,6 +273,19 @@ class EVSL:
         if err:
             return err, None, None, None
         return (0,) + self._collect(n, nevo.value, lam, W, res)
+
+    def cheb_av_cols(self, pols, X, prec=PREC_FP64):
+        """Y[:, j] = p_j(A) X[:, j] with the blocked filter (X: n x ncols host
+        array; pols: one PolParams per column, same cc and dd).  Returns
+        (err, Y)."""
+        X = np.asfortranarray(X, dtype=np.float64)
+        ncols = X.shape[1]
+        assert len(pols) == ncols
+        P = (PolParams * ncols)(*pols)
+        Y = np.zeros_like(X, order="F")
+        err = self.lib.evsl_amd_cheb_av_cols(ncols, P, X.ctypes.data_as(c_double_p),
+                                             Y.ctypes.data_as(c_double_p), prec)
+        return err, Y
 
     def ChebLanNr_batch(self, intvs, maxit, tol, vinit, pols, n=None, fstats=None):
         """All slices of one matrix in one batched run.  intvs: list of

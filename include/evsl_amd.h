@@ -53,6 +53,8 @@ int evsl_amd_set_refine_guard(double guard);
 
 int evsl_amd_get_stats(evslAmdStats *s);
 void evsl_amd_reset_stats(void);
+/* Message of the first failure in the most recent device call ("" if it
+ * succeeded; host-only calls leave it unchanged). */
 const char *evsl_amd_last_error(void);
 /* size in bytes of evslAmdStats (ABI check for bindings) */
 size_t evsl_amd_stats_size(void);
@@ -74,6 +76,15 @@ int kpmdos_ex(int Mdeg, int damping, int nvec, double *intv, int vec_kind,
 int ChebLanNr_batch(int nslices, double *intvs, int maxit, double tol, double *vinit,
                     polparams *pols, int *nevOut, double **lamo, double **Wo, double **reso,
                     FILE *fstats);
+
+/* Y(:, j) = p_j(A) X(:, j) for the ncols columns of the n x ncols
+ * column-major X with the blocked filter (the block form of the reference's
+ * internal ChebAv, SRC/chebpoly.c:519-598): prec EVSL_PREC_FP64, or
+ * EVSL_PREC_MIXED for the FP32 filter on the shifted matrix.  pols[j] (from
+ * find_pol) may differ per column but must share cc and dd.  X and Y may be
+ * host or device memory. */
+int evsl_amd_cheb_av_cols(int ncols, const polparams *pols, const double *X, double *Y,
+                          int prec);
 
 /* Matrix Market coordinate reader of the reference drivers
  * (TESTS/COMMON/io.c:115-217, read_coo_MM): real/integer, general/symmetric
