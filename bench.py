@@ -287,6 +287,10 @@ def main():
                                   if args.form != "plain" and st["t_filter"] else None),
         "breakdown_s": {"filter": st["t_filter"], "reorth": st["t_reorth"],
                         "ritz_refine": st["t_ritz"]} if args.form != "plain" else None,
+        "counts": {"reorth_passes": int(st["n_reorth_pass"]),
+                   "refine_columns": int(st["n_refine_cols"]),
+                   "refine_extra_passes": int(st["n_refine_extra"]),
+                   "refine_GB": st["refine_bytes"] / 1e9} if args.form != "plain" else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kname, "launch_ms": ms, "bytes_per_launch": by,

@@ -74,11 +74,17 @@ int block_filter(int B, const FilterOp& op, const double* mus_d, int maxdeg, con
   return block_filter_seg(B, op, mus_d, 1, maxdeg, X0, bufs, Y);
 }
 
-// Shared work buffers of the blocked filter (sized for Bmax columns).
+// Shared work buffers of the blocked filter (sized for Bmax columns).  The
+// coefficient tables stay on the device while the same filters come back
+// (the batched solver filters the same slices step after step), so a filter
+// pass issues no host-device copy and no synchronisation.
 struct BlockWork {
-  DevBuf X0, bufs, Y, Y2, zero, mus, colptr_in, colptr_out;
+  DevBuf X0, bufs, Y, Y2, zero, junk, mus;
   int Bcap = 0;
   bool f32 = false;
+  std::vector<const polparams*> mus_key;  // filters (in column order) of `mus`
+  int mus_kind = -1;                      // layout of `mus`: B, or 0 for phased
+  std::vector<size_t> mus_off;            // phased: offset of each phase's table
   int ensure(int n, int B, bool fp32, int maxdeg) {
     const size_t vs = fp32 ? 4 : 8;
     const size_t nb = (size_t)n * B * vs;
@@ -87,9 +93,7 @@ struct BlockWork {
     EVSL_HIP(Y.alloc(nb));
     EVSL_HIP(Y2.alloc(nb));
     EVSL_HIP(zero.alloc((size_t)n * sizeof(double)));
-    EVSL_HIP(mus.alloc((size_t)(maxdeg + 1) * B * sizeof(double)));
-    EVSL_HIP(colptr_in.alloc(B * sizeof(double*)));
-    EVSL_HIP(colptr_out.alloc(B * sizeof(double*)));
+    EVSL_HIP(junk.alloc((size_t)n * sizeof(double)));
     if (B > Bcap) {
       EVSL_HIP(hipMemsetAsync(zero.p, 0, (size_t)n * sizeof(double), ctx().stream));
       Bcap = B;
@@ -100,12 +104,26 @@ struct BlockWork {
   void* buf(int i, int n, int B) const {
     return bufs.as<char>() + (size_t)i * n * B * (f32 ? 4 : 8);
   }
+  // uploads `tab` as the coefficient table of (key, kind) unless it is there
+  template <typename Build>
+  int coef_table(const std::vector<const polparams*>& key, int kind, Build build) {
+    if (kind == mus_kind && key == mus_key) return 0;
+    std::vector<double> tab;
+    mus_off.clear();
+    build(tab, mus_off);
+    // the copy below is ordered after the launches that read the old table
+    EVSL_HIP(mus.alloc(tab.size() * sizeof(double)));
+    EVSL_HIP(hipMemcpy(mus.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    mus_key = key;
+    mus_kind = kind;
+    return 0;
+  }
 };
 
-int upload_ptrs(DevBuf& dst, const void* const* p, int cnt) {
-  EVSL_HIP(dst.alloc((size_t)cnt * sizeof(void*)));
-  EVSL_HIP(hipMemcpy(dst.p, p, (size_t)cnt * sizeof(void*), hipMemcpyHostToDevice));
-  return 0;
+evsl_dev::ColPtrs col_ptrs(const double* const* p, int cnt) {
+  evsl_dev::ColPtrs cp{};
+  for (int i = 0; i < cnt; ++i) cp.p[i] = const_cast<double*>(p[i]);
+  return cp;
 }
 
 // Columns of different degrees (b <= 8, every degree >= 3): ordered by
@@ -126,62 +144,78 @@ int filter_columns_phased(int b, const double* const* xin, double* const* yout,
   if (int e = W.ensure(n, b, f32, md)) return e;
   std::vector<const double*> in(b);
   std::vector<double*> out(b);
+  std::vector<const polparams*> key(b);
   for (int i = 0; i < b; ++i) {
     in[i] = xin[ord[i]];
     out[i] = yout[ord[i]];
+    key[i] = pols[ord[i]];
   }
-  if (int e = upload_ptrs(W.colptr_in, (const void* const*)in.data(), b)) return e;
-  if (int e = upload_ptrs(W.colptr_out, (const void* const*)out.data(), b)) return e;
-  EVSL_HIP(evsl_dev::pack_cols(n, b, W.colptr_in.as<const double*>(), W.X0.p, f32, c.stream));
+  // phase p runs degrees up to the smallest degree of its active columns;
+  // its table holds coefficient rows 0..kend for those columns
+  auto phases = [&](auto&& visit) {
+    int active = b, k0 = 1;
+    while (active > 0) {
+      const int kend = key[active - 1]->deg;
+      int keep = active;
+      while (keep > 0 && key[keep - 1]->deg == kend) --keep;
+      visit(active, k0, kend, keep);
+      active = keep;
+      k0 = kend + 1;
+    }
+  };
+  if (int e = W.coef_table(key, 0, [&](std::vector<double>& tab, std::vector<size_t>& off) {
+        phases([&](int active, int, int kend, int) {
+          off.push_back(tab.size());
+          tab.resize(tab.size() + (size_t)(kend + 1) * active, 0.0);
+          double* t = tab.data() + off.back();
+          for (int i = 0; i < active; ++i)
+            for (int d = 0; d <= kend; ++d) t[(size_t)d * active + i] = key[i]->mu[d];
+        });
+      }))
+    return e;
+  EVSL_HIP(evsl_dev::pack_cols(n, b, col_ptrs(in.data(), b), W.X0.p, f32, c.stream));
   FilterOp op;
-  if (int e = make_filter_op(pols[ord[0]]->cc, pols[ord[0]]->dd, f32, &op)) return e;
+  if (int e = make_filter_op(key[0]->cc, key[0]->dd, f32, &op)) return e;
   void* slot[3] = {W.buf(0, n, b), W.buf(1, n, b), W.buf(2, n, b)};
   void* Y = W.Y.p;
   void* Yspare = W.Y2.p;
-  double* const* outp = W.colptr_out.as<double* const>();
   const size_t vs = f32 ? 4 : 8;
-  int active = b, k0 = 1;
-  std::vector<double> mus;
-  while (active > 0) {
-    const int kend = pols[ord[active - 1]]->deg;
-    // coefficient rows 0..kend for the active columns
-    mus.assign((size_t)(kend + 1) * active, 0.0);
-    for (int i = 0; i < active; ++i)
-      for (int d = 0; d <= kend; ++d) mus[(size_t)d * active + i] = pols[ord[i]]->mu[d];
-    EVSL_HIP(hipMemcpy(W.mus.p, mus.data(), mus.size() * sizeof(double), hipMemcpyHostToDevice));
-    if (int e = block_filter_seg(active, op, W.mus.as<double>(), k0, kend, W.X0.p, slot, Y))
-      return e;
-    int keep = active;
-    while (keep > 0 && pols[ord[keep - 1]]->deg == kend) --keep;
-    EVSL_HIP(evsl_dev::unpack_range(n, active, keep, active, Y, f32, outp + keep, c.stream));
-    if (keep > 0) {
+  int phase = 0, err = 0;
+  phases([&](int active, int k0, int kend, int keep) {
+    if (err) return;
+    const double* mus_d = W.mus.as<double>() + W.mus_off[phase++];
+    if ((err = block_filter_seg(active, op, mus_d, k0, kend, W.X0.p, slot, Y))) return;
+    hipError_t he = evsl_dev::unpack_range(n, active, keep, active, Y, f32,
+                                           col_ptrs(out.data() + keep, active - keep), c.stream);
+    if (he == hipSuccess && keep > 0) {
       // T_kend sits in slot (kend-1)%3 and T_{kend-1} in slot (kend-2)%3
       // (kend >= 3); narrow them into the free slots and swap the pointers
       const int sa = (kend - 1) % 3, sb = (kend - 2) % 3, sf = kend % 3;
       void* pa = slot[sa];
       void* pb = slot[sb];
       void* pf = slot[sf];
-      EVSL_HIP(evsl_dev::narrow_cols(n, active, keep, pa, pf, f32, c.stream));
-      EVSL_HIP(evsl_dev::narrow_cols(n, active, keep, pb, pa, f32, c.stream));
+      he = evsl_dev::narrow_cols(n, active, keep, pa, pf, f32, c.stream);
+      if (he == hipSuccess) he = evsl_dev::narrow_cols(n, active, keep, pb, pa, f32, c.stream);
       slot[sa] = pf;
       slot[sb] = pa;
       slot[sf] = pb;
-      EVSL_HIP(evsl_dev::narrow_cols(n, active, keep, Y, Yspare, f32, c.stream));
+      if (he == hipSuccess) he = evsl_dev::narrow_cols(n, active, keep, Y, Yspare, f32, c.stream);
       std::swap(Y, Yspare);
       c.astats.filter_bytes += 3.0 * (double)n * (active + keep) * vs;
     }
-    active = keep;
-    k0 = kend + 1;
-  }
-  // the pointer tables are reused by the next chunk: wait for this one
-  EVSL_HIP(hipStreamSynchronize(c.stream));
-  return 0;
+    if (he != hipSuccess) {
+      set_error("filter_columns_phased: unpack/narrow", he);
+      err = EVSL_ERR_DEVICE;
+    }
+  });
+  return err;
 }
 
 // y_c = p_c(A) x_c for columns given by device pointers (FP64 in/out), using
 // the interleaved block filter, at most 32 columns per launch.  pols[c] may
 // differ per column (same cc, dd).  Blocks wider than 8 are padded to 16 or
-// 32 with zero columns whose output goes to a scratch vector.
+// 32 with zero columns whose output goes to a scratch vector.  Nothing here
+// waits for the device.
 int filter_columns(int ncols, const double* const* xin, double* const* yout,
                    const polparams* const* pols, bool f32, BlockWork& W) {
   Context& c = ctx();
@@ -194,26 +228,22 @@ int filter_columns(int ncols, const double* const* xin, double* const* yout,
   }
   int maxdeg = 0;
   for (int j = 0; j < ncols; ++j) maxdeg = std::max(maxdeg, pols[j]->deg);
-  DevBuf junk;
   for (int j0 = 0; j0 < ncols; j0 += kBmaxPow) {
     const int b = std::min(kBmaxPow, ncols - j0);
     const int B = kernel_width(b);
     if (int e = W.ensure(n, B, f32, maxdeg)) return e;
-    if (B > b && !junk.p) EVSL_HIP(junk.alloc((size_t)n * sizeof(double)));
     std::vector<const double*> in(B);
     std::vector<double*> out(B);
-    std::vector<double> mus((size_t)(maxdeg + 1) * B, 0.0);
+    std::vector<const polparams*> key(pols + j0, pols + j0 + b);
     int md = 0;
     for (int cc = 0; cc < B; ++cc) {
       if (cc < b) {
-        const polparams* p = pols[j0 + cc];
         in[cc] = xin[j0 + cc];
         out[cc] = yout[j0 + cc];
-        for (int d = 0; d <= p->deg; ++d) mus[(size_t)d * B + cc] = p->mu[d];
-        md = std::max(md, p->deg);
+        md = std::max(md, pols[j0 + cc]->deg);
       } else {
         in[cc] = W.zero.as<double>();
-        out[cc] = junk.as<double>();
+        out[cc] = W.junk.as<double>();
       }
     }
     if (md == 0) {  // degree-0 filters: y = mu0 x
@@ -231,17 +261,18 @@ int filter_columns(int ncols, const double* const* xin, double* const* yout,
       if (int e = filter_columns_phased(b, xin + j0, yout + j0, pols + j0, f32, W)) return e;
       continue;
     }
-    EVSL_HIP(hipMemcpy(W.mus.p, mus.data(), mus.size() * sizeof(double), hipMemcpyHostToDevice));
-    if (int e = upload_ptrs(W.colptr_in, (const void* const*)in.data(), B)) return e;
-    if (int e = upload_ptrs(W.colptr_out, (const void* const*)out.data(), B)) return e;
-    EVSL_HIP(evsl_dev::pack_cols(n, B, W.colptr_in.as<const double*>(), W.X0.p, f32, c.stream));
+    if (int e = W.coef_table(key, B, [&](std::vector<double>& tab, std::vector<size_t>&) {
+          tab.assign((size_t)(md + 1) * B, 0.0);
+          for (int cc = 0; cc < b; ++cc)
+            for (int d = 0; d <= key[cc]->deg; ++d) tab[(size_t)d * B + cc] = key[cc]->mu[d];
+        }))
+      return e;
+    EVSL_HIP(evsl_dev::pack_cols(n, B, col_ptrs(in.data(), B), W.X0.p, f32, c.stream));
     void* bufs[3] = {W.buf(0, n, B), W.buf(1, n, B), W.buf(2, n, B)};
     FilterOp op;
     if (int e = make_filter_op(pols[j0]->cc, pols[j0]->dd, f32, &op)) return e;
     if (int e = block_filter(B, op, W.mus.as<double>(), md, W.X0.p, bufs, W.Y.p)) return e;
-    EVSL_HIP(evsl_dev::unpack_cols(n, B, W.Y.p, f32, W.colptr_out.as<double* const>(), c.stream));
-    // the pointer tables are reused by the next chunk: wait for this one
-    EVSL_HIP(hipStreamSynchronize(c.stream));
+    EVSL_HIP(evsl_dev::unpack_cols(n, B, W.Y.p, f32, col_ptrs(out.data(), B), c.stream));
   }
   return 0;
 }
@@ -285,27 +316,22 @@ int apply_A_cols(int n, int m, const double* U, double* W, BlockWork& Wk) {
     return 0;
   }
   const evsl_dev::Csr A = c.A.view();
-  DevBuf junk;
   for (int j0 = 0; j0 < m; j0 += kBmaxPow) {
     const int b = std::min(kBmaxPow, m - j0);
     const int B = kernel_width(b);
     if (int e = Wk.ensure(n, B, false, 0)) return e;
-    if (B > b && !junk.p) EVSL_HIP(junk.alloc((size_t)n * sizeof(double)));
     std::vector<const double*> in(B);
     std::vector<double*> out(B);
     for (int cc = 0; cc < B; ++cc) {
       in[cc] = cc < b ? U + (size_t)(j0 + cc) * n : Wk.zero.as<double>();
-      out[cc] = cc < b ? W + (size_t)(j0 + cc) * n : junk.as<double>();
+      out[cc] = cc < b ? W + (size_t)(j0 + cc) * n : Wk.junk.as<double>();
     }
-    if (int e = upload_ptrs(Wk.colptr_in, (const void* const*)in.data(), B)) return e;
-    if (int e = upload_ptrs(Wk.colptr_out, (const void* const*)out.data(), B)) return e;
     void* X = Wk.buf(0, n, B);
     void* Yb = Wk.buf(1, n, B);
-    EVSL_HIP(evsl_dev::pack_cols(n, B, Wk.colptr_in.as<const double*>(), X, false, c.stream));
+    EVSL_HIP(evsl_dev::pack_cols(n, B, col_ptrs(in.data(), B), X, false, c.stream));
     EVSL_HIP(evsl_dev::cheb_spmm(A, B, false, evsl_dev::kPlain, X, nullptr, Yb, nullptr, nullptr,
                                  nullptr, nullptr, 1.0, 0.0, false, 0, nullptr, c.stream));
-    EVSL_HIP(evsl_dev::unpack_cols(n, B, Yb, false, Wk.colptr_out.as<double* const>(), c.stream));
-    EVSL_HIP(hipStreamSynchronize(c.stream));
+    EVSL_HIP(evsl_dev::unpack_cols(n, B, Yb, false, col_ptrs(out.data(), B), c.stream));
   }
   return 0;
 }
@@ -360,47 +386,51 @@ int residual_filter(int n, int m, double* U, double* W, const polparams* pol, Bl
   EVSL_HIP(hipMemcpy(pv.p, pth.data(), m * sizeof(double), hipMemcpyHostToDevice));
   FilterOp op;
   if (int e = make_filter_op(pol->cc, pol->dd, true, &op)) return e;
-  DevBuf junk, g_d, mp_d;
-  for (int j0 = 0; j0 < m; j0 += kBmaxPow) {
+  // per-launch coefficient tables of every chunk, uploaded once:
+  // g[chunk][k][c] = 2 T_k(theta~_c), k = 1..deg-1; pairs (mu_{k+1}, mu_1)
+  const int nchunk = (m + kBmaxPow - 1) / kBmaxPow;
+  const size_t gstride = (size_t)(deg + 1) * kBmaxPow;
+  std::vector<double> g(gstride * nchunk, 0.0), mp((size_t)2 * (deg + 1), 0.0);
+  for (int j0 = 0, ch = 0; j0 < m; j0 += kBmaxPow, ++ch) {
+    const int b = std::min(kBmaxPow, m - j0);
+    const int B = kernel_width(b);
+    for (int cc = 0; cc < b; ++cc) {
+      const double x = ths[j0 + cc];
+      double tkm1 = 1.0, tk = x;
+      for (int k = 1; k < deg; ++k) {
+        g[gstride * ch + (size_t)k * B + cc] = 2.0 * tk;
+        const double tkp1 = 2.0 * x * tk - tkm1;
+        tkm1 = tk;
+        tk = tkp1;
+      }
+    }
+  }
+  for (int k = 1; k < deg; ++k) {
+    mp[2 * k] = pol->mu[k + 1];
+    mp[2 * k + 1] = pol->mu[1];
+  }
+  DevBuf g_d, mp_d;
+  EVSL_HIP(g_d.alloc(g.size() * sizeof(double)));
+  EVSL_HIP(mp_d.alloc(mp.size() * sizeof(double)));
+  EVSL_HIP(hipMemcpy(g_d.p, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice));
+  EVSL_HIP(hipMemcpy(mp_d.p, mp.data(), mp.size() * sizeof(double), hipMemcpyHostToDevice));
+  for (int j0 = 0, ch = 0; j0 < m; j0 += kBmaxPow, ++ch) {
     const int b = std::min(kBmaxPow, m - j0);
     const int B = kernel_width(b);
     if (int e = bw.ensure(n, B, true, deg)) return e;
-    if (B > b && !junk.p) EVSL_HIP(junk.alloc((size_t)n * sizeof(double)));
     // R columns -> FP32 block (xk of the first launch = D_1 = R); the same
     // block is the v0 operand (R) of every launch
     std::vector<const double*> in(B);
     std::vector<double*> out(B);
     for (int cc = 0; cc < B; ++cc) {
       in[cc] = cc < b ? W + (size_t)(j0 + cc) * n : bw.zero.as<double>();
-      out[cc] = cc < b ? W + (size_t)(j0 + cc) * n : junk.as<double>();
+      out[cc] = cc < b ? W + (size_t)(j0 + cc) * n : bw.junk.as<double>();
     }
-    if (int e = upload_ptrs(bw.colptr_in, (const void* const*)in.data(), B)) return e;
-    if (int e = upload_ptrs(bw.colptr_out, (const void* const*)out.data(), B)) return e;
-    EVSL_HIP(evsl_dev::pack_cols(n, B, bw.colptr_in.as<const double*>(), bw.X0.p, true, c.stream));
-    // per-launch coefficient tables: g[k][c] = 2 T_k(theta~_c), k = 1..deg-1;
-    // pairs (mu_{k+1}, mu_1)
-    std::vector<double> g((size_t)(deg + 1) * B, 0.0), mp((size_t)2 * (deg + 1), 0.0);
-    for (int cc = 0; cc < b; ++cc) {
-      const double x = ths[j0 + cc];
-      double tkm1 = 1.0, tk = x;
-      for (int k = 1; k < deg; ++k) {
-        g[(size_t)k * B + cc] = 2.0 * tk;
-        const double tkp1 = 2.0 * x * tk - tkm1;
-        tkm1 = tk;
-        tk = tkp1;
-      }
-    }
-    for (int k = 1; k < deg; ++k) {
-      mp[2 * k] = pol->mu[k + 1];
-      mp[2 * k + 1] = pol->mu[1];
-    }
-    EVSL_HIP(g_d.alloc(g.size() * sizeof(double)));
-    EVSL_HIP(mp_d.alloc(mp.size() * sizeof(double)));
-    EVSL_HIP(hipMemcpy(g_d.p, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice));
-    EVSL_HIP(hipMemcpy(mp_d.p, mp.data(), mp.size() * sizeof(double), hipMemcpyHostToDevice));
+    EVSL_HIP(evsl_dev::pack_cols(n, B, col_ptrs(in.data(), B), bw.X0.p, true, c.stream));
     void* X0 = bw.X0.p;  // R (FP32) = D_1
     void* bufs[3] = {bw.buf(0, n, B), bw.buf(1, n, B), bw.buf(2, n, B)};
     void* Yb = bw.Y.p;
+    const double* gch = g_d.as<double>() + gstride * ch;
     if (deg == 1) {
       // Y = mu_1 R (W already holds R in FP64)
       for (int cc = 0; cc < b; ++cc)
@@ -412,15 +442,14 @@ int residual_filter(int n, int m, double* U, double* W, const polparams* pol, Bl
         const void* xkm1 = (k == 1) ? nullptr : (k == 2 ? X0 : bufs[(k - 3) % 3]);
         const int yf = evsl_dev::y_flush(k, deg - 1);
         EVSL_HIP(evsl_dev::cheb_spmm(op.A, B, true, evsl_dev::kRes, xk, xkm1, xkp1, Yb, X0,
-                                     g_d.as<double>() + (size_t)k * B, mp_d.as<double>() + 2 * k,
-                                     op.t2, op.cc, k == 1, yf, nullptr, c.stream));
+                                     gch + (size_t)k * B, mp_d.as<double>() + 2 * k, op.t2,
+                                     op.cc, k == 1, yf, nullptr, c.stream));
         // the kCheb streams plus the residual block R
         c.astats.refine_bytes += (double)c.A.nnz * (4 + c.A.idx_bytes()) + 4.0 * (n + 1) +
                                  (double)(evsl_dev::cheb_streams(yf, k == 1) + 1) * B * n * 4;
       }
-      EVSL_HIP(evsl_dev::unpack_cols(n, B, Yb, true, bw.colptr_out.as<double* const>(), c.stream));
+      EVSL_HIP(evsl_dev::unpack_cols(n, B, Yb, true, col_ptrs(out.data(), B), c.stream));
     }
-    EVSL_HIP(hipStreamSynchronize(c.stream));
   }
   // U <- U p(theta~) + Y
   EVSL_HIP(evsl_dev::col_scale_add(n, m, U, n, pv.as<double>(), W, n, c.stream));
@@ -824,20 +853,33 @@ int ChebLanNr_batch(int nslices, double* intvs, int maxit, double tol, double* v
     c.astats.n_lanczos += b;
     // ---- convergence tests (restricted trace, every 20 steps from 30) --------
     if ((k < Ntest || (k - Ntest) % cycle != 0) && k != maxit - 1) continue;
-    for (int s : act) {
-      Slice& L = *S[s];
-      const int kdim = k + 1;
+    // the slices' tridiagonal eigenvalue problems are independent: one host
+    // thread each (the device is idle until the next filter pass)
+    const int kdim = k + 1;
+    std::vector<double> tr1s(b, 0.0);
+    std::vector<int> nconvs(b, 0), qlerr(b, 0);
+#pragma omp parallel for schedule(dynamic, 1) if (b > 1)
+    for (int j = 0; j < b; ++j) {
+      const Slice& L = *S[act[j]];
       std::vector<double> ev(kdim), e(kdim + 1, 0.0);
       std::memcpy(ev.data(), L.dT.data(), kdim * sizeof(double));
       std::memcpy(e.data(), L.eT.data(), (kdim - 1) * sizeof(double));
-      if (tridiag_ql(kdim, ev.data(), e.data(), nullptr)) return EVSL_ERR_EIG;
-      double tr1 = 0.0;
-      int nconv = 0;
+      if (tridiag_ql(kdim, ev.data(), e.data(), nullptr)) {
+        qlerr[j] = 1;
+        continue;
+      }
       for (int i = 0; i < kdim; ++i)
         if (ev[i] + kEpsM >= L.pol->bar) {
-          tr1 += ev[i];
-          ++nconv;
+          tr1s[j] += ev[i];
+          ++nconvs[j];
         }
+    }
+    for (int j = 0; j < b; ++j) {
+      const int s = act[j];
+      Slice& L = *S[s];
+      if (qlerr[j]) return EVSL_ERR_EIG;
+      const double tr1 = tr1s[j];
+      const int nconv = nconvs[j];
       // FP64: the reference's test.  Mixed: the filtered operator is known to
       // ~1e-6, so the trace is compared at that level and the count of Ritz
       // values above bar must also be unchanged.

@@ -182,19 +182,23 @@ hipError_t rm_col_scale_rsqrt(size_t n, int B, double* X, const double* s2,
 hipError_t rm_random(size_t n, int B, int kind, unsigned long long seed, int col0,
                      int ncols, double* X, hipStream_t st);
 
-// Interleave B column vectors (device pointers cols[0..B)) into a row-major
-// block X (r*B + c), converting to float if fp32; and the inverse (scaled by
-// scale[c] if scale != nullptr).
-hipError_t pack_cols(size_t n, int B, const double* const* cols, void* X, bool fp32,
-                     hipStream_t st);
+// Device addresses of up to kMaxCols FP64 column vectors, handed to the
+// pack/unpack kernels by value (a kernel argument: no device table to upload)
+constexpr int kMaxCols = 32;
+struct ColPtrs {
+  double* p[kMaxCols];
+};
+// Interleave B column vectors cols.p[0..B) into a row-major block X
+// (r*B + c), converting to float if fp32; and the inverse.
+hipError_t pack_cols(size_t n, int B, const ColPtrs& cols, void* X, bool fp32, hipStream_t st);
 // dst (n x B2) = the first B2 columns of the interleaved n x B block src
 // (separate buffers)
 hipError_t narrow_cols(size_t n, int B, int B2, const void* src, void* dst, bool fp32,
                        hipStream_t st);
-// cols[c - c0] (FP64 device vectors) = column c of the interleaved block, c0 <= c < c1
+// cols.p[c - c0] (FP64 device vectors) = column c of the interleaved block, c0 <= c < c1
 hipError_t unpack_range(size_t n, int B, int c0, int c1, const void* X, bool fp32,
-                        double* const* cols, hipStream_t st);
-hipError_t unpack_cols(size_t n, int B, const void* X, bool fp32, double* const* cols,
+                        const ColPtrs& cols, hipStream_t st);
+hipError_t unpack_cols(size_t n, int B, const void* X, bool fp32, const ColPtrs& cols,
                        hipStream_t st);
 // row-major block <-> column-major panel (ld) of B columns starting at col0
 hipError_t rm_to_cm(size_t n, int B, const double* X, double* P, size_t ld, hipStream_t st);

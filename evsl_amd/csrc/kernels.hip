@@ -776,24 +776,22 @@ __global__ void k_rm_random(size_t n, int B, int kind, unsigned long long seed, 
 }
 
 template <typename T>
-__global__ void k_pack_cols(size_t n, int B, const double* const* __restrict__ cols,
-                            T* __restrict__ X) {
+__global__ void k_pack_cols(size_t n, int B, ColPtrs cols, T* __restrict__ X) {
   const size_t tot = n * (size_t)B;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
        i += (size_t)gridDim.x * blockDim.x) {
     const int c = (int)(i % B);
-    X[i] = (T)cols[c][i / B];
+    X[i] = (T)cols.p[c][i / B];
   }
 }
 
 template <typename T>
-__global__ void k_unpack_cols(size_t n, int B, const T* __restrict__ X,
-                              double* const* __restrict__ cols) {
+__global__ void k_unpack_cols(size_t n, int B, const T* __restrict__ X, ColPtrs cols) {
   const size_t tot = n * (size_t)B;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
        i += (size_t)gridDim.x * blockDim.x) {
     const int c = (int)(i % B);
-    cols[c][i / B] = (double)X[i];
+    cols.p[c][i / B] = (double)X[i];
   }
 }
 
@@ -809,17 +807,17 @@ __global__ void k_narrow_cols(size_t n, int B, int B2, const T* __restrict__ src
   }
 }
 
-// cols[c - c0][r] = X[r*B + c] for the columns c0 <= c < c1 of an interleaved block
+// cols.p[c - c0][r] = X[r*B + c] for the columns c0 <= c < c1 of an interleaved block
 template <typename T>
 __global__ void k_unpack_range(size_t n, int B, int c0, int c1, const T* __restrict__ X,
-                               double* const* __restrict__ cols) {
+                               ColPtrs cols) {
   const int w = c1 - c0;
   const size_t tot = n * (size_t)w;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
        i += (size_t)gridDim.x * blockDim.x) {
     const size_t r = i / w;
     const int c = (int)(i - r * w);
-    cols[c][r] = (double)X[r * B + c0 + c];
+    cols.p[c][r] = (double)X[r * B + c0 + c];
   }
 }
 
@@ -1343,8 +1341,8 @@ hipError_t rm_random(size_t n, int B, int kind, unsigned long long seed, int col
   return hipGetLastError();
 }
 
-hipError_t pack_cols(size_t n, int B, const double* const* cols, void* X, bool fp32,
-                     hipStream_t st) {
+hipError_t pack_cols(size_t n, int B, const ColPtrs& cols, void* X, bool fp32, hipStream_t st) {
+  if (B <= 0 || B > kMaxCols) return hipErrorInvalidValue;
   const unsigned g = grid_for(n * B, kWG, 8192);
   if (fp32)
     hipLaunchKernelGGL(k_pack_cols<float>, dim3(g), dim3(kWG), 0, st, n, B, cols, (float*)X);
@@ -1353,8 +1351,9 @@ hipError_t pack_cols(size_t n, int B, const double* const* cols, void* X, bool f
   return hipGetLastError();
 }
 
-hipError_t unpack_cols(size_t n, int B, const void* X, bool fp32, double* const* cols,
+hipError_t unpack_cols(size_t n, int B, const void* X, bool fp32, const ColPtrs& cols,
                        hipStream_t st) {
+  if (B <= 0 || B > kMaxCols) return hipErrorInvalidValue;
   const unsigned g = grid_for(n * B, kWG, 8192);
   if (fp32)
     hipLaunchKernelGGL(k_unpack_cols<float>, dim3(g), dim3(kWG), 0, st, n, B, (const float*)X,
@@ -1379,8 +1378,8 @@ hipError_t narrow_cols(size_t n, int B, int B2, const void* src, void* dst, bool
 }
 
 hipError_t unpack_range(size_t n, int B, int c0, int c1, const void* X, bool fp32,
-                        double* const* cols, hipStream_t st) {
-  if (c0 < 0 || c1 > B || c0 >= c1) return hipErrorInvalidValue;
+                        const ColPtrs& cols, hipStream_t st) {
+  if (c0 < 0 || c1 > B || c0 >= c1 || c1 - c0 > kMaxCols) return hipErrorInvalidValue;
   const unsigned grid = grid_for(n * (size_t)(c1 - c0), kWG, 8192);
   if (fp32)
     hipLaunchKernelGGL(k_unpack_range<float>, dim3(grid), dim3(kWG), 0, st, n, B, c0, c1,
