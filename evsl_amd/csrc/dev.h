@@ -88,6 +88,9 @@ int cheb_spmm_rows(int B, bool fp32, Mode mode);
 // number of row blocks (KPM partials per column) one cheb_spmm launch of
 // this width, precision and mode uses; -1 for an unsupported width
 int cheb_spmm_nblk(const Csr& A, int B, bool fp32, Mode mode);
+// whether a launch at (B, precision, mode) on A uses the non-temporal
+// instance (k_cheb_spmm<..., NT = true>)
+bool cheb_spmm_streaming(const Csr& A, int B, bool fp32, Mode mode);
 
 // out[c] = sum_b part[b*B + c] over nblk row blocks, in a fixed order; uses
 // scratch (>= kSumScratch doubles) for a first level when nblk is large

@@ -63,7 +63,20 @@ struct alignas(sizeof(VT) * V) VecT {
 // (the L lanes of a row load the same address).  The per-column summation
 // order is the same for every (V, LDS), so results are bitwise independent
 // of the variant (the KPM block partials aside, whose tree depends on NG).
-template <typename VT, int B, int V, int MODE, bool LDS, bool I16>
+// NT marks the streams a launch touches last -- xkm1, the flushed y and the
+// xkp1 store -- non-temporal, so that they do not push the gathered x rows
+// out of L2; it pays when the launch's working set exceeds the Infinity
+// Cache (chosen per launch, streaming_launch below) and leaves the
+// arithmetic unchanged.
+template <typename T, int V>
+struct ExtVec {
+  typedef T type __attribute__((ext_vector_type(V)));
+};
+template <typename T>
+struct ExtVec<T, 1> {
+  typedef T type;
+};
+template <typename VT, int B, int V, int MODE, bool LDS, bool I16, bool NT = false>
 __global__ __launch_bounds__(kWG) void k_cheb_spmm(
     const int* __restrict__ ia, const int* __restrict__ ja, const short* __restrict__ jd,
     const VT* __restrict__ av,
@@ -137,14 +150,35 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
   struct Ops {
     Vec xr, km1, yo, rr, v;
   };
+  using EV = typename ExtVec<VT, V>::type;
   auto ld = [](const VT* p, size_t i) { return *reinterpret_cast<const Vec*>(p + i); };
+  // the same load / store, non-temporal when NT
+  auto ld_last = [](const VT* p, size_t i) {
+    if constexpr (NT) {
+      const EV e = __builtin_nontemporal_load(reinterpret_cast<const EV*>(p + i));
+      Vec r;
+      __builtin_memcpy(&r, &e, sizeof(Vec));
+      return r;
+    } else {
+      return *reinterpret_cast<const Vec*>(p + i);
+    }
+  };
+  auto st_last = [](VT* p, size_t i, const Vec& v) {
+    if constexpr (NT) {
+      EV e;
+      __builtin_memcpy(&e, &v, sizeof(Vec));
+      __builtin_nontemporal_store(e, reinterpret_cast<EV*>(p + i));
+    } else {
+      *reinterpret_cast<Vec*>(p + i) = v;
+    }
+  };
   auto prefetch = [&](int r) {
     Ops o{};
     const size_t i = (size_t)r * B + c0;
     if (MODE == kPlain) return o;
     if (need_xr) o.xr = ld(xk, i);
-    if (xkm1) o.km1 = ld(xkm1, i);
-    if (ymode && yflush > 0 && !first) o.yo = ld(y, i);
+    if (xkm1) o.km1 = ld_last(xkm1, i);
+    if (ymode && yflush > 0 && !first) o.yo = ld_last(y, i);
     if (MODE == kRes) o.rr = ld(v0, i);
     if (MODE == kKpm) o.v = ld(v0, i);
     return o;
@@ -162,7 +196,7 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
       if (MODE == kRes) wj += g[j] * o.rr.v[j];
       w.v[j] = wj;
     }
-    *reinterpret_cast<Vec*>(xkp1 + i) = w;
+    st_last(xkp1, i, w);
     if (ymode) {
       if (yflush > 0) {
         // the same additions, in the same order, as one update per degree
@@ -174,7 +208,7 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
           if (yflush >= 2) yv = yv + s1[j] * o.xr.v[j];
           yn.v[j] = yv + s[j] * w.v[j];
         }
-        *reinterpret_cast<Vec*>(y + i) = yn;
+        st_last(y, i, yn);
       }
     } else if (MODE == kKpm) {
 #pragma unroll
@@ -923,6 +957,28 @@ inline unsigned grid_for(size_t n, int per_block, unsigned cap) {
   return (unsigned)g;
 }
 
+// Non-temporal streams (the NT instances) exist for the ChebAv and residual
+// steps of 8 or more columns.  They are chosen when one launch's working set
+// -- the matrix, and the vector blocks xk, xkm1, xkp1, y (and R) -- exceeds
+// 320 MB: beyond the 256 MB Infinity Cache the block no longer stays
+// resident from launch to launch, and keeping the last-use streams out of L2
+// saves the x rows re-read across planes.  Measured on the 100^3 Laplacian
+// (µs per launch, plain -> non-temporal): FP32 32-column ChebAv 137.5 -> 124.8
+// and residual step 149.0 -> 141.4, FP32 16-column residual 81.9 -> 71.4, FP64
+// 8/16/32-column ChebAv 71.5 -> 67.8, 131.5 -> 118.3, 242.2 -> 232.5, FP64
+// 8-column residual 85.0 -> 72.7; below the threshold the hints cost 3-30%
+// (FP32 8- and 16-column ChebAv at 174 / 302 MB, FP64 4-column at 197 MB).
+template <int MODE, int B>
+constexpr bool kStreamingCapable = (MODE == kCheb || MODE == kRes) && B >= 8;
+
+inline bool streaming_launch(const Csr& A, int B, int vsize, int mode) {
+  if (!((mode == kCheb || mode == kRes) && B >= 8)) return false;
+  const double streams = mode == kRes ? 5.0 : 4.0;
+  const double ws = streams * (double)A.n * B * vsize +
+                    (double)A.nnz * (vsize + (A.jd ? 2 : 4)) + 4.0 * (A.n + 1);
+  return ws > 320e6;
+}
+
 // One fused step with an explicit lane layout (V columns per lane, LDS
 // staging on/off, R rows per lane group); the row-block table is the one
 // built for B/V/R lanes per row.
@@ -941,16 +997,25 @@ hipError_t launch_variant(const Csr& A, const void* xk, const void* xkm1, void* 
   if (slot < 0 || !A.rb[slot]) return hipErrorInvalidValue;
   const int nblk = A.nblk[slot];
   const int2* rb = reinterpret_cast<const int2*>(A.rb[slot]);
+  const int f = first ? 1 : 0;
+#define EVSL_SPMM_LAUNCH(I16_, NT_)                                                             \
+  hipLaunchKernelGGL((k_cheb_spmm<VT, B, V, MODE, LDS, I16_, NT_>), dim3(nblk), dim3(kWG), 0, \
+                     st, A.ia, A.ja, A.jd, av, rb, nblk, (const VT*)xk, (const VT*)xkm1,      \
+                     (VT*)xkp1, (VT*)y, (const VT*)v0, s, mu0, t, cc, f, yflush, part)
+  if constexpr (kStreamingCapable<MODE, B>) {
+    if (streaming_launch(A, B, (int)sizeof(VT), MODE)) {
+      if (A.jd)
+        EVSL_SPMM_LAUNCH(true, true);
+      else
+        EVSL_SPMM_LAUNCH(false, true);
+      return hipGetLastError();
+    }
+  }
   if (A.jd)
-    hipLaunchKernelGGL((k_cheb_spmm<VT, B, V, MODE, LDS, true>), dim3(nblk), dim3(kWG), 0, st,
-                       A.ia, A.ja, A.jd, av, rb, nblk, (const VT*)xk, (const VT*)xkm1,
-                       (VT*)xkp1, (VT*)y, (const VT*)v0, s, mu0, t, cc, first ? 1 : 0, yflush,
-                       part);
+    EVSL_SPMM_LAUNCH(true, false);
   else
-    hipLaunchKernelGGL((k_cheb_spmm<VT, B, V, MODE, LDS, false>), dim3(nblk), dim3(kWG), 0, st,
-                       A.ia, A.ja, A.jd, av, rb, nblk, (const VT*)xk, (const VT*)xkm1,
-                       (VT*)xkp1, (VT*)y, (const VT*)v0, s, mu0, t, cc, first ? 1 : 0, yflush,
-                       part);
+    EVSL_SPMM_LAUNCH(false, false);
+#undef EVSL_SPMM_LAUNCH
   return hipGetLastError();
 }
 
@@ -1092,6 +1157,10 @@ int cheb_spmm_vec(int B, bool fp32, Mode mode) { return policy_factor(B, fp32, m
 int cheb_spmm_rows(int B, bool fp32, Mode mode) {
   const int v = policy_factor(B, fp32, mode, true), vr = policy_factor(B, fp32, mode, false);
   return v > 0 ? vr / v : -1;
+}
+
+bool cheb_spmm_streaming(const Csr& A, int B, bool fp32, Mode mode) {
+  return streaming_launch(A, B, fp32 ? 4 : 8, mode);
 }
 
 int cheb_spmm_nblk(const Csr& A, int B, bool fp32, Mode mode) {

@@ -307,6 +307,16 @@ int evsl_amd_spmm_vec(int B, int prec, int mode) {
   return evsl_dev::cheb_spmm_vec(B, prec == EVSL_PREC_MIXED && mode != 2, md);
 }
 
+int evsl_amd_spmm_streaming(int B, int prec, int mode) {
+  const Context& c = ctx();
+  if (c.op != 1) return 0;
+  const evsl_dev::Mode md =
+      mode == 2 ? evsl_dev::kKpm : (mode == 3 ? evsl_dev::kRes : evsl_dev::kCheb);
+  return evsl_dev::cheb_spmm_streaming(c.A.view(), B, prec == EVSL_PREC_MIXED && mode != 2, md)
+             ? 1
+             : 0;
+}
+
 int evsl_amd_bench_copy(size_t bytes, int reps, double* ms_per_copy) {
   if (int e = ensure_started()) return e;
   if (reps <= 0 || bytes < 16 || !ms_per_copy) return EVSL_ERR_ARG;

@@ -42,6 +42,7 @@ EXPORTS = [
     "evsl_amd_last_error", "evsl_amd_stats_size", "kpmdos_ex", "evsl_amd_bench_filter",
     "ChebLanNr_batch", "evsl_amd_read_coo_mm", "evsl_amd_bench_copy",
     "evsl_amd_spmm_vec", "evsl_amd_spmm_index_bytes", "evsl_amd_cheb_av_cols",
+    "evsl_amd_spmm_streaming",
 ]
 
 
@@ -122,6 +123,7 @@ class EVSL:
             "evsl_amd_bench_copy": (I, [C.c_size_t, I, dp]),
             "evsl_amd_spmm_vec": (I, [I, I, I]),
             "evsl_amd_spmm_index_bytes": (I, []),
+            "evsl_amd_spmm_streaming": (I, [I, I, I]),
             "evsl_amd_cheb_av_cols": (I, [I, pp, dp, dp, I]),
             "ChebLanNr_batch": (I, [I, dp, I, D, dp, pp, ip, ddp, ddp, ddp, VP]),
         }
@@ -377,4 +379,6 @@ class EVSL:
         the current matrix, as rocprofv3 reports it."""
         vt = "float" if prec == PREC_MIXED and mode != 2 else "double"
         i16 = "true" if self.lib.evsl_amd_spmm_index_bytes() == 2 else "false"
-        return f"k_cheb_spmm<{vt}, {B}, {self.spmm_vec(B, prec, mode)}, {mode}, true, {i16}>"
+        nt = "true" if self.lib.evsl_amd_spmm_streaming(B, prec, mode) else "false"
+        return (f"k_cheb_spmm<{vt}, {B}, {self.spmm_vec(B, prec, mode)}, {mode}, true, {i16}, "
+                f"{nt}>")

@@ -104,8 +104,14 @@ int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, doub
 
 /* Columns per lane of the fused filter kernel evsl_amd_bench_filter times
  * for (B, prec, mode): its template instance is
- * k_cheb_spmm<float|double, B, V, mode, true>; -1 for an unsupported B. */
+ * k_cheb_spmm<float|double, B, V, mode, true, I16, NT> (I16 from
+ * evsl_amd_spmm_index_bytes, NT from evsl_amd_spmm_streaming); -1 for an
+ * unsupported B. */
 int evsl_amd_spmm_vec(int B, int prec, int mode);
+/* 1 if the fused filter kernel for (B, prec, mode) on the current matrix
+ * runs its non-temporal instance (k_cheb_spmm<..., I16, true>: one launch's
+ * working set exceeds 320 MB), 0 otherwise or without a matrix. */
+int evsl_amd_spmm_streaming(int B, int prec, int mode);
 /* Bytes per nonzero of the column index the filter kernels read for the
  * current matrix: 2 when every |ja - row| fits in 16 bits (the kernels then
  * run as k_cheb_spmm<..., true, true>), 4 otherwise; 0 without a matrix. */

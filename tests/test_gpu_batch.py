@@ -34,6 +34,9 @@ def setup_slices(ev, A, a, b, lo, hi, nslices, nvec=40, mdeg=300):
 
 CASES = {
     "lap2d": lambda: (M.laplacian(100, 100, 1), 0.4, 0.8, 0.0, 8.0, 2),
+    # 16 slices of different degrees: a 16-column block (padded series) until
+    # slices finish, then the degree-phased narrowing below 8 columns
+    "lap2d_16slices": lambda: (M.laplacian(100, 100, 1), 0.3, 1.5, 0.0, 8.0, 16),
     "lap3d": lambda: (M.perturb_diag(M.laplacian(30, 30, 30), 1e-3, seed=5), 0.5, 0.6, None, None, 4),
     "anderson": lambda: (M.anderson(24, 16.5, seed=4), -0.1, 0.1, None, None, 4),
     "randsym": lambda: (M.random_symmetric(20000, 8, 32, seed=3), -0.2, 0.2, None, None, 3),

@@ -113,3 +113,33 @@ def test_filter_cols_rejects_bad_arguments(ev):
     assert err == -104
     ev.free_pol(p1)
     ev.free_pol(p2)
+
+
+def test_filter_nontemporal_instances_bitwise(ev):
+    """On the 88^3 Laplacian one launch's working set passes 320 MB at 32
+    FP32 columns and at 16 and 32 FP64 columns, so those widths run the
+    fused kernel's non-temporal instance; its columns must equal the
+    one-column runs (plain instance) bit for bit."""
+    A = M.perturb_diag(M.laplacian(88, 88, 88), 1e-3, seed=4)
+    assert ev.set_matrix(A) == 0, ev.last_error()
+    lib = ev.lib
+    assert lib.evsl_amd_spmm_streaming(32, PREC_MIXED, 1) == 1
+    assert lib.evsl_amd_spmm_streaming(8, PREC_MIXED, 1) == 0
+    assert lib.evsl_amd_spmm_streaming(16, PREC_FP64, 1) == 1
+    assert lib.evsl_amd_spmm_streaming(4, PREC_FP64, 1) == 0
+    pg = make_pols(ev, [(0.4, 0.7), (1.5, 1.9), (5.0, 6.0)], 0.0, 12.001)
+    ncol = 32
+    X = np.random.default_rng(5).uniform(-1.0, 1.0, (A.n, ncol))
+    which = [j % 3 for j in range(ncol)]
+    for prec in (PREC_FP64, PREC_MIXED):
+        Y1 = np.empty((A.n, ncol))
+        for j in range(ncol):
+            err, y = ev.cheb_av_cols([pg[which[j]]], X[:, j:j + 1], prec)
+            assert err == 0, ev.last_error()
+            Y1[:, j] = y[:, 0]
+        for w in (16, 32):
+            err, Y = ev.cheb_av_cols([pg[which[j]] for j in range(w)], X[:, :w], prec)
+            assert err == 0, (w, prec, ev.last_error())
+            assert np.array_equal(Y, Y1[:, :w]), (prec, w, np.abs(Y - Y1[:, :w]).max())
+    for p in pg:
+        ev.free_pol(p)
