@@ -92,9 +92,49 @@ int cheb_spmm_nblk(const Csr& A, int B, bool fp32, Mode mode);
 // instance (k_cheb_spmm<..., NT = true>)
 bool cheb_spmm_streaming(const Csr& A, int B, bool fp32, Mode mode);
 
+// One matrix of a grouped KPM launch (a device-resident table entry): its
+// CSR (int32 column indices), its row-block table for the launch's lane
+// layout ((row, ia[row]) pairs), the start block v0 and the ring of three
+// T_k blocks (n x B interleaved, element type of the launch), the block
+// partials of the moment dots, and the recurrence
+// T_{k+1} = t (A - cc I) T_k - T_{k-1} with factor t0 at k = 0.
+struct KpmGroupEntry {
+  const int* ia;
+  const int* ja;
+  const void* a;
+  const int* rbp;
+  const void* x0;
+  void* ring;
+  double* part;
+  double t0, t, cc;
+  int n, blk0;  // rows; first global row block of this matrix
+};
+// The partial sums of one grouped matrix: nblk block partials (B per block)
+// summed as sum_block_partials does (G first-level blocks into scratch, then
+// one; or directly when G == 1) into dots + (k+1)*B.
+struct KpmSumEntry {
+  const double* part;
+  double* scratch;
+  double* dots;
+  int nblk, G;
+};
+// One grouped KPM step k over `total` row blocks (owner[gb] = entry of row
+// block gb) at block width B (lane layout of the kKpm policy), then its
+// partial sums: level 1 over nl1 blocks (owner1/first1), level 2 over the
+// two-level entries listed in owner2.
+hipError_t kpm_group_step(bool fp32, int B, const KpmGroupEntry* tab, const int* owner,
+                          int total, int k, const KpmSumEntry* sums, const int* owner1,
+                          const int* first1, int nl1, const int* owner2, int nl2,
+                          hipStream_t st);
+
 // out[c] = sum_b part[b*B + c] over nblk row blocks, in a fixed order; uses
 // scratch (>= kSumScratch doubles) for a first level when nblk is large
 constexpr int kSumScratch = 256 * 32;
+// first-level blocks of sum_block_partials for (nblk, B): 1 when it sums
+// directly, else kSumScratch / 32
+inline int sum_levels_blocks(int nblk, int B) {
+  return ((size_t)nblk * B <= 16384) ? 1 : kSumScratch / 32;
+}
 hipError_t sum_block_partials(const double* part, int nblk, int B, double* out,
                               double* scratch, hipStream_t st);
 

@@ -169,6 +169,10 @@ int check_intv(const double* intv);
 // device pointer for n doubles the caller holds on the host or the device
 // (a host array is copied into tmp)
 int as_device_vector(const double* p, size_t n, DevBuf& tmp, const double** out);
+// KPM moments and eigenvalue count from raw[k] = sum over the nvec vectors of
+// v^T T_k(A~) v (k >= 1), as kpmdos finishes them
+void kpm_moments(int Mdeg, int damping, int nvec, const double* intv, int n, const double* raw,
+                 double* mu, double* ecnt);
 // workspace of one filtered Lanczos run
 struct LanWork {
   DevBuf V, gt_part, h, vrand;

@@ -76,13 +76,15 @@ template <typename T>
 struct ExtVec<T, 1> {
   typedef T type;
 };
-template <typename VT, int B, int V, int MODE, bool LDS, bool I16, bool NT = false>
-__global__ __launch_bounds__(kWG) void k_cheb_spmm(
-    const int* __restrict__ ia, const int* __restrict__ ja, const short* __restrict__ jd,
-    const VT* __restrict__ av,
-    const int2* __restrict__ rbp, int nblk, const VT* __restrict__ xk,
-    const VT* __restrict__ xkm1, VT* __restrict__ xkp1, VT* __restrict__ y,
-    const VT* __restrict__ v0, const double* __restrict__ scol,
+// The work of one row block `blk`: shared by the one-matrix kernel
+// (k_cheb_spmm) and the grouped KPM kernel (k_kpm_group), so that a block's
+// arithmetic is the same in both.
+template <typename VT, int B, int V, int MODE, bool LDS, bool I16, bool NT>
+__device__ __forceinline__ void cheb_spmm_block(
+    const int blk, const int* __restrict__ ia, const int* __restrict__ ja,
+    const short* __restrict__ jd, const VT* __restrict__ av, const int2* __restrict__ rbp,
+    const VT* __restrict__ xk, const VT* __restrict__ xkm1, VT* __restrict__ xkp1,
+    VT* __restrict__ y, const VT* __restrict__ v0, const double* __restrict__ scol,
     const double* __restrict__ mu0col, double t_, double cc_, int first, int yflush,
     double* __restrict__ part) {
   static_assert(B % V == 0, "V must divide B");
@@ -106,7 +108,6 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
   VT* s_a = reinterpret_cast<VT*>(smem + (size_t)kNnzCap * sizeof(IT));
   double* s_red = reinterpret_cast<double*>(smem);
 
-  const int blk = xcd_remap(blockIdx.x, nblk);
   // block table entries carry (first row, its row pointer): one load for both ends
   const int2 e0 = rbp[blk], e1 = rbp[blk + 1];
   const int r0 = e0.x, r1 = e1.x, p0 = e0.y, p1 = e1.y;
@@ -325,18 +326,54 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
   }
 }
 
-// out[c] = sum over blocks b in [b0, b1) of part[b*B + c]; grid-strided
-// over block ranges when called with gridDim.x > 1 (then out = scratch of
-// gridDim.x * B values)
-__global__ __launch_bounds__(kWG) void k_sum_block_partials(const double* __restrict__ part,
-                                                            int nblk, int B,
-                                                            double* __restrict__ out) {
-  __shared__ double s_red[kWG];
+template <typename VT, int B, int V, int MODE, bool LDS, bool I16, bool NT = false>
+__global__ __launch_bounds__(kWG) void k_cheb_spmm(
+    const int* __restrict__ ia, const int* __restrict__ ja, const short* __restrict__ jd,
+    const VT* __restrict__ av,
+    const int2* __restrict__ rbp, int nblk, const VT* __restrict__ xk,
+    const VT* __restrict__ xkm1, VT* __restrict__ xkp1, VT* __restrict__ y,
+    const VT* __restrict__ v0, const double* __restrict__ scol,
+    const double* __restrict__ mu0col, double t_, double cc_, int first, int yflush,
+    double* __restrict__ part) {
+  cheb_spmm_block<VT, B, V, MODE, LDS, I16, NT>(xcd_remap(blockIdx.x, nblk), ia, ja, jd, av, rbp,
+                                                xk, xkm1, xkp1, y, v0, scol, mu0col, t_, cc_,
+                                                first, yflush, part);
+}
+
+// KPM steps of several matrices in one launch: global row block gb (after
+// the XCD remap) belongs to matrix owner[gb], whose entry names its CSR, its
+// row-block table, its vector blocks and its recurrence factors; step k
+// reads T_k and T_{k-1} from the matrix's ring and writes T_{k+1} and the
+// block's moment partials, as the one-matrix kKpm launch does.
+template <typename VT, int B, int V>
+__global__ __launch_bounds__(kWG) void k_kpm_group(const KpmGroupEntry* __restrict__ tab,
+                                                   const int* __restrict__ owner, int total,
+                                                   int k) {
+  const int gb = xcd_remap(blockIdx.x, total);
+  const KpmGroupEntry& e = tab[owner[gb]];
+  const size_t nb = (size_t)e.n * B;
+  const VT* x0 = static_cast<const VT*>(e.x0);
+  VT* ring = static_cast<VT*>(e.ring);
+  const VT* xk = k == 0 ? x0 : ring + (size_t)((k - 1) % 3) * nb;
+  const VT* xkm1 = k == 0 ? nullptr : (k == 1 ? x0 : ring + (size_t)((k - 2) % 3) * nb);
+  VT* xkp1 = ring + (size_t)(k % 3) * nb;
+  cheb_spmm_block<VT, B, V, kKpm, true, false, false>(
+      gb - e.blk0, e.ia, e.ja, nullptr, static_cast<const VT*>(e.a),
+      reinterpret_cast<const int2*>(e.rbp), xk, xkm1, xkp1, nullptr, x0, nullptr, nullptr,
+      k == 0 ? e.t0 : e.t, e.cc, 0, 0, e.part);
+}
+
+// out[c] = sum over blocks b in [b0, b1) of part[b*B + c] for block `bid` of
+// `nb` (the range split evenly over the nb blocks; out = scratch of nb * B
+// values when nb > 1)
+__device__ __forceinline__ void sum_partials_block(const double* __restrict__ part, int nblk,
+                                                   int B, double* __restrict__ out, int bid,
+                                                   int nb, double* s_red) {
   const int NG = kWG / B;
   const int col = threadIdx.x % B, grp = threadIdx.x / B;
   const bool act = (int)threadIdx.x < NG * B;
-  const int per = (nblk + gridDim.x - 1) / gridDim.x;
-  const int b0 = blockIdx.x * per, b1 = min(nblk, b0 + per);
+  const int per = (nblk + nb - 1) / nb;
+  const int b0 = bid * per, b1 = min(nblk, b0 + per);
   double a = 0.0;
   if (act)
     for (int b = b0 + grp; b < b1; b += NG) a += part[(size_t)b * B + col];
@@ -345,7 +382,36 @@ __global__ __launch_bounds__(kWG) void k_sum_block_partials(const double* __rest
   if (act && grp == 0) {
     double t = 0.0;
     for (int g = 0; g < NG; ++g) t += s_red[g * B + col];
-    out[(size_t)blockIdx.x * B + col] = t;
+    out[(size_t)bid * B + col] = t;
+  }
+}
+
+__global__ __launch_bounds__(kWG) void k_sum_block_partials(const double* __restrict__ part,
+                                                            int nblk, int B,
+                                                            double* __restrict__ out) {
+  __shared__ double s_red[kWG];
+  sum_partials_block(part, nblk, B, out, blockIdx.x, gridDim.x, s_red);
+}
+
+// The partial sums of a grouped KPM step, per matrix in the order of
+// sum_block_partials: level 1 runs entry owner[blockIdx] as block
+// blockIdx - first[owner] of its G; level 2 sums the G first-level values of
+// each two-level entry.
+__global__ __launch_bounds__(kWG) void k_kpm_group_sums(const KpmSumEntry* __restrict__ tab,
+                                                        const int* __restrict__ owner,
+                                                        const int* __restrict__ first, int B,
+                                                        int k, int level) {
+  __shared__ double s_red[kWG];
+  const int r = owner[blockIdx.x];
+  const KpmSumEntry& e = tab[r];
+  double* out = e.dots + (size_t)(k + 1) * B;
+  if (level == 1) {
+    if (e.G == 1)
+      sum_partials_block(e.part, e.nblk, B, out, 0, 1, s_red);
+    else
+      sum_partials_block(e.part, e.nblk, B, e.scratch, blockIdx.x - first[r], e.G, s_red);
+  } else {
+    sum_partials_block(e.scratch, e.G, B, out, 0, 1, s_red);
   }
 }
 
@@ -1202,11 +1268,44 @@ hipError_t cheb_spmm(const Csr& A, int B, bool fp32, Mode mode, const void* xk,
   return hipErrorInvalidValue;
 }
 
+hipError_t kpm_group_step(bool fp32, int B, const KpmGroupEntry* tab, const int* owner,
+                          int total, int k, const KpmSumEntry* sums, const int* owner1,
+                          const int* first1, int nl1, const int* owner2, int nl2,
+                          hipStream_t st) {
+  if (total <= 0 || nl1 <= 0) return hipErrorInvalidValue;
+#define EVSL_KPM_GROUP(VT_, B_)                                                           \
+  hipLaunchKernelGGL((k_kpm_group<VT_, B_, SpmmPolicy<VT_, B_, kKpm>::V>), dim3(total),   \
+                     dim3(kWG), 0, st, tab, owner, total, k)
+#define EVSL_KPM_GROUP_B(VT_)                  \
+  switch (B) {                                 \
+    case 1: EVSL_KPM_GROUP(VT_, 1); break;     \
+    case 2: EVSL_KPM_GROUP(VT_, 2); break;     \
+    case 4: EVSL_KPM_GROUP(VT_, 4); break;     \
+    case 8: EVSL_KPM_GROUP(VT_, 8); break;     \
+    case 16: EVSL_KPM_GROUP(VT_, 16); break;   \
+    case 32: EVSL_KPM_GROUP(VT_, 32); break;   \
+    default: return hipErrorInvalidValue;      \
+  }
+  if (fp32) {
+    EVSL_KPM_GROUP_B(float);
+  } else {
+    EVSL_KPM_GROUP_B(double);
+  }
+#undef EVSL_KPM_GROUP_B
+#undef EVSL_KPM_GROUP
+  hipLaunchKernelGGL(k_kpm_group_sums, dim3(nl1), dim3(kWG), 0, st, sums, owner1, first1, B, k,
+                     1);
+  if (nl2 > 0)
+    hipLaunchKernelGGL(k_kpm_group_sums, dim3(nl2), dim3(kWG), 0, st, sums, owner2,
+                       (const int*)nullptr, B, k, 2);
+  return hipGetLastError();
+}
+
 hipError_t sum_block_partials(const double* part, int nblk, int B, double* out,
                               double* scratch, hipStream_t st) {
   if (B < 1 || B > 32) return hipErrorInvalidValue;
   const int G = kSumScratch / 32;  // first-level blocks (<= 256)
-  if ((size_t)nblk * B <= 16384 || !scratch) {
+  if (sum_levels_blocks(nblk, B) == 1 || !scratch) {
     hipLaunchKernelGGL(k_sum_block_partials, dim3(1), dim3(kWG), 0, st, part, nblk, B, out);
   } else {
     hipLaunchKernelGGL(k_sum_block_partials, dim3(G), dim3(kWG), 0, st, part, nblk, B, scratch);

@@ -20,6 +20,31 @@
 #include "internal.h"
 
 namespace evsl {
+
+// mu_k = (2 - delta_k0) jac_k raw_k / (nvec pi) with raw_0 = nvec, and the
+// eigenvalue count n * integral of the KPM density over [a, b] ∩ [lmin, lmax]
+// (SRC/spslice.c:57-72, 120-146)
+void kpm_moments(int Mdeg, int damping, int nvec, const double* intv, int n, const double* raw,
+                 double* mu, double* ecnt) {
+  const double aa = std::max(intv[0], intv[2]), bb = std::min(intv[1], intv[3]);
+  const double ctr = (intv[3] + intv[2]) / 2.0, wid = (intv[3] - intv[2]) / 2.0;
+  const double beta1 = std::acos(std::max(-1.0 + DBL_EPSILON, (aa - ctr) / wid));
+  const double beta2 = std::acos(std::min(1.0 - DBL_EPSILON, (bb - ctr) / wid));
+  std::vector<double> jac(Mdeg + 1);
+  dampcf(Mdeg, damping, jac.data());
+  jac[0] = 1.0;
+  double tcnt = -(double)nvec * jac[0] * (beta2 - beta1);
+  mu[0] = (double)nvec * jac[0];
+  for (int k = 1; k <= Mdeg; ++k) {
+    const double t = 2 * jac[k] * raw[k];
+    mu[k] = t;
+    tcnt -= t * (std::sin(k * beta2) - std::sin(k * beta1)) / k;
+  }
+  const double t = 1.0 / (((double)nvec) * EVSL_PI);
+  for (int k = 0; k <= Mdeg; ++k) mu[k] *= t;
+  *ecnt = tcnt * t * (double)n;
+}
+
 namespace {
 
 int kpm_impl(int Mdeg, int damping, int nvec, const double* intv, int kind,
@@ -30,13 +55,7 @@ int kpm_impl(int Mdeg, int damping, int nvec, const double* intv, int kind,
   if (intv[0] >= intv[1] || intv[0] >= intv[3] || intv[1] <= intv[2]) return -1;
   if (Mdeg < 0 || nvec <= 0) return EVSL_ERR_ARG;
   const int n = c.n;
-  const double aa = std::max(intv[0], intv[2]), bb = std::min(intv[1], intv[3]);
   const double ctr = (intv[3] + intv[2]) / 2.0, wid = (intv[3] - intv[2]) / 2.0;
-  const double beta1 = std::acos(std::max(-1.0 + DBL_EPSILON, (aa - ctr) / wid));
-  const double beta2 = std::acos(std::min(1.0 - DBL_EPSILON, (bb - ctr) / wid));
-  std::vector<double> jac(Mdeg + 1);
-  dampcf(Mdeg, damping, jac.data());
-  jac[0] = 1.0;
   std::vector<double> raw(Mdeg + 1, 0.0);  // raw[k] = sum_m v_m . T_k v_m
   const int Bmax = 32;
   DevBuf X0, X, dots;
@@ -124,16 +143,7 @@ int kpm_impl(int Mdeg, int damping, int nvec, const double* intv, int kind,
       for (int col = 0; col < B; ++col) raw[k] += hdots[(size_t)k * B + col];
     done += B;
   }
-  double tcnt = -(double)nvec * jac[0] * (beta2 - beta1);
-  mu[0] = (double)nvec * jac[0];
-  for (int k = 1; k <= Mdeg; ++k) {
-    const double t = 2 * jac[k] * raw[k];
-    mu[k] = t;
-    tcnt -= t * (std::sin(k * beta2) - std::sin(k * beta1)) / k;
-  }
-  const double t = 1.0 / (((double)nvec) * EVSL_PI);
-  for (int k = 0; k <= Mdeg; ++k) mu[k] *= t;
-  *ecnt = tcnt * t * (double)n;
+  kpm_moments(Mdeg, damping, nvec, intv, n, raw.data(), mu, ecnt);
   return 0;
 }
 

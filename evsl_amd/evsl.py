@@ -42,7 +42,7 @@ EXPORTS = [
     "evsl_amd_last_error", "evsl_amd_stats_size", "kpmdos_ex", "evsl_amd_bench_filter",
     "ChebLanNr_batch", "evsl_amd_read_coo_mm", "evsl_amd_bench_copy",
     "evsl_amd_spmm_vec", "evsl_amd_spmm_index_bytes", "evsl_amd_cheb_av_cols",
-    "evsl_amd_spmm_streaming",
+    "evsl_amd_spmm_streaming", "kpmdos_batch",
 ]
 
 
@@ -124,6 +124,8 @@ class EVSL:
             "evsl_amd_spmm_vec": (I, [I, I, I]),
             "evsl_amd_spmm_index_bytes": (I, []),
             "evsl_amd_spmm_streaming": (I, [I, I, I]),
+            "kpmdos_batch": (I, [I, C.POINTER(C.POINTER(CsrMat)), dp, I, I, I, I,
+                                 C.POINTER(C.c_ulonglong), I, dp, dp]),
             "evsl_amd_cheb_av_cols": (I, [I, pp, dp, dp, I]),
             "ChebLanNr_batch": (I, [I, dp, I, D, dp, pp, ip, ddp, ddp, ddp, VP]),
         }
@@ -288,6 +290,25 @@ class EVSL:
         err = self.lib.evsl_amd_cheb_av_cols(ncols, P, X.ctypes.data_as(c_double_p),
                                              Y.ctypes.data_as(c_double_p), prec)
         return err, Y
+
+    def kpmdos_batch(self, mats, intvs, Mdeg, damping, nvec, vec_kind=VEC_RADEMACHER,
+                     seeds=None, prec=PREC_FP64):
+        """KPM densities of several matrices in one batched run.  mats: list
+        of HostCsr (repeats allowed), intvs: list of 4-vectors.  Returns
+        (err, mu [nreq x (Mdeg+1)], ecnt [nreq])."""
+        nreq = len(mats)
+        P = (C.POINTER(CsrMat) * nreq)(*[C.pointer(A.csr) for A in mats])
+        iv = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64)
+                                                  for x in intvs]))
+        sd = None
+        if seeds is not None:
+            sd = (C.c_ulonglong * nreq)(*[int(x) for x in seeds])
+        mu = np.zeros((nreq, Mdeg + 1))
+        ecnt = np.zeros(nreq)
+        err = self.lib.kpmdos_batch(nreq, P, dptr(iv), Mdeg, damping, nvec, vec_kind, sd, prec,
+                                    mu.ctypes.data_as(c_double_p),
+                                    ecnt.ctypes.data_as(c_double_p))
+        return err, mu, ecnt
 
     def ChebLanNr_batch(self, intvs, maxit, tol, vinit, pols, n=None, fstats=None):
         """All slices of one matrix in one batched run.  intvs: list of

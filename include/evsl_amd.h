@@ -64,6 +64,22 @@ size_t evsl_amd_stats_size(void);
 int kpmdos_ex(int Mdeg, int damping, int nvec, double *intv, int vec_kind,
               unsigned long long seed, double *mu, double *ecnt);
 
+/* KPM densities of several matrices in one batched run (requests of uneven
+ * size advance together: one grouped launch per Chebyshev degree over the
+ * row blocks of every matrix).  Request r is kpmdos (SRC/spslice.c:29-148)
+ * on mats[r] (host CSR; the same pointer may appear in several requests)
+ * with interval intvs[4r..4r+3], Mdeg, damping and nvec shared, and the
+ * seeded device vectors of kpmdos_ex (vec_kind EVSL_VEC_RADEMACHER or
+ * EVSL_VEC_GAUSSIAN, seed seeds[r], or r when seeds is NULL): in
+ * EVSL_PREC_FP64 its moments equal kpmdos_ex on that matrix bit for bit;
+ * EVSL_PREC_MIXED runs the recurrence in FP32 on (A - ctr I) 2^-s with the
+ * moment dots accumulated in FP64.  Outputs mu[r*(Mdeg+1) + k], ecnt[r].
+ * Independent of SetAMatrix.  Returns 0, -1 for an invalid interval,
+ * EVSL_ERR_ARG, EVSL_ERR_UNSUPP (vec_kind) or a device error. */
+int kpmdos_batch(int nreq, csrMat *const *mats, const double *intvs, int Mdeg, int damping,
+                 int nvec, int vec_kind, const unsigned long long *seeds, int prec, double *mu,
+                 double *ecnt);
+
 /* All slices of one matrix as one batched filtered Lanczos run.  Slice s is
  * [intvs[4s], intvs[4s+1]] with global bounds intvs[4s+2..4s+3] (identical
  * for every slice) and filter pols[s] (from find_pol).  Each slice follows
