@@ -157,18 +157,111 @@ def cpu_baseline(A, lo, hi, step, args):
     del Q
     est = args.mdeg * args.nvec * t_kpm_deg
     avg_steps = step["stats"]["n_lanczos"] / max(1, len(step["slices"]))
+    # CGS passes per Lanczos step as the GPU run measured them (DGKS adds a
+    # second pass only when the first leaves too little of the vector)
+    passes = step["stats"]["n_reorth_pass"] / max(1, step["stats"]["n_lanczos"])
     for s in step["slices"]:
         steps = s.get("steps", avg_steps)
         est += (steps + 1) * s["deg"] * t_deg
-        est += 1.5 * t_col * steps * (steps + 1) / 2.0   # ~1.5 DGKS passes per step
+        est += passes * t_col * steps * (steps + 1) / 2.0
     sample_secs = t_kpm_deg * kdeg + t_lan + t_col * k
     total_pairs = sum(s["nev"] for s in step["slices"])
     return {"value": total_pairs / est, "unit": "eigenpairs/s", "cores": 1, "kind": "port",
             "sample": (f"oracle (C, -O2, 1 thread) on the same matrix: {kdeg} KPM degrees, "
                        f"{lan_steps} filtered Lanczos steps of slice 0 (deg {s0['deg']}), 1 CGS "
                        f"pass vs {k} vectors ({sample_secs:.1f}s); full step extrapolated to "
-                       f"{est:.0f}s from the GPU run's per-slice Lanczos steps and degrees"),
+                       f"{est:.0f}s from the GPU run's per-slice Lanczos steps, degrees and "
+                       f"CGS passes per step ({passes:.2f})"),
             "extrapolated_step_s": est}
+
+
+def kpm_batch_bench(args):
+    """Batched KPM form: args.kb_requests DOS requests (Mdeg 200, 32 Rademacher
+    vectors, Jackson damping) over a pool of args.kb_matrices matrices with n
+    log-uniform in [2^14, 2^20] (--kb-log2n) -- 2-D and 3-D Laplacians and random
+    symmetric matrices with 8-32 nonzeros per row -- each request with its
+    own interval and seed.  Times the batched call (FP32 recurrence with FP64
+    moment sums, and FP64) against the same requests one matrix at a time
+    (kpmdos_ex after SetAMatrix), and checks FP64 batch == one at a time."""
+    from evsl_amd import matrices as M
+    from evsl_amd.evsl import EVSL, PREC_FP64, PREC_MIXED, VEC_RADEMACHER
+    rng = np.random.default_rng(4242)
+    mdeg, nvec = 200, 32
+    pool = []
+    t0 = time.perf_counter()
+    for i in range(args.kb_matrices):
+        n = int(2 ** rng.uniform(*args.kb_log2n))
+        if i % 3 == 0:
+            s2 = max(2, int(round(n ** 0.5)))
+            A = M.laplacian(s2, s2, 1)
+        elif i % 3 == 1:
+            s3 = max(2, int(round(n ** (1 / 3))))
+            A = M.laplacian(s3, s3, s3)
+        else:
+            A = M.random_symmetric(n, 8, 32, seed=int(rng.integers(1 << 30)))
+        lo, hi = M.gershgorin(A)
+        pool.append((A, lo, hi))
+        log(f"[bench] kpm batch: matrix {i}: n {A.n}, nnz {A.nnz} "
+            f"({time.perf_counter() - t0:.1f}s)")
+    mats, ivs, seeds, owner = [], [], [], []
+    for r in range(args.kb_requests):
+        m = r % len(pool)
+        A, lo, hi = pool[m]
+        a = lo + (hi - lo) * rng.uniform(0.05, 0.85)
+        ivs.append(np.array([a, a + 0.1 * (hi - lo), lo, hi]))
+        mats.append(A)
+        seeds.append(1000 + r)
+        owner.append(m)
+    sum_n = sum(A.n for A in mats)
+    ev = EVSL()
+    assert ev.start() == 0
+    ev.kpmdos_batch(mats[:2], ivs[:2], 4, 1, 2, VEC_RADEMACHER, seeds[:2])  # priming
+    res = {}
+    for name, prec in (("mixed", PREC_MIXED), ("fp64", PREC_FP64)):
+        t0 = time.perf_counter()
+        err, mu, ecnt = ev.kpmdos_batch(mats, ivs, mdeg, 1, nvec, VEC_RADEMACHER, seeds, prec)
+        res[name] = (time.perf_counter() - t0, mu, ecnt)
+        assert err == 0, ev.last_error()
+        log(f"[bench] kpm batch {name}: {res[name][0]:.2f}s")
+    # one matrix at a time (FP64 kpmdos_ex per request; uploads timed apart)
+    t_seq, t_set = 0.0, 0.0
+    same = True
+    for m, (A, lo, hi) in enumerate(pool):
+        t0 = time.perf_counter()
+        assert ev.set_matrix(A) == 0
+        t_set += time.perf_counter() - t0
+        for r in [r for r in range(len(mats)) if owner[r] == m]:
+            t0 = time.perf_counter()
+            e1, mu1, ec1 = ev.kpmdos(mdeg, 1, nvec, ivs[r], vec_kind=VEC_RADEMACHER,
+                                     seed=seeds[r])
+            t_seq += time.perf_counter() - t0
+            same = same and e1 == 0 and np.array_equal(mu1, res["fp64"][1][r])
+    log(f"[bench] kpm one at a time: {t_seq:.2f}s (+{t_set:.2f}s SetAMatrix)")
+    vs = {"mixed": 4, "fp64": 8}
+    gbs = {}
+    for name in ("mixed", "fp64"):
+        b = sum(A.nnz * (vs[name] + 4) + 4 * (A.n + 1) + 4 * A.n * nvec * vs[name]
+                for A in mats) * mdeg
+        gbs[name] = b / res[name][0] / 1e9
+    d = np.abs(res["mixed"][1] - res["fp64"][1])
+    print(json.dumps({
+        "workload": f"batched KPM: {len(mats)} DOS requests over {len(pool)} matrices "
+                    f"(2-D/3-D Laplacians, random symmetric 8-32 nnz/row), n log-uniform in "
+                    f"[2^{args.kb_log2n[0]:g}, 2^{args.kb_log2n[1]:g}], Mdeg {mdeg}, {nvec} "
+                    f"Rademacher vectors, Jackson",
+        "sum_n": int(sum_n), "requests": len(mats),
+        "batched_s": {k: res[k][0] for k in res},
+        "requests_per_s": {k: len(mats) / res[k][0] for k in res},
+        "algorithmic_GBps_incl_upload": gbs,
+        "one_at_a_time_fp64_s": t_seq, "set_matrix_s": t_set,
+        "speedup_fp64_batched_vs_one_at_a_time": t_seq / res["fp64"][0],
+        "speedup_mixed_batched_vs_fp64_one_at_a_time": t_seq / res["mixed"][0],
+        "fp64_batched_equals_one_at_a_time": bool(same),
+        "mixed_vs_fp64_max_abs_moment_diff": float(d.max()),
+        "mixed_vs_fp64_max_rel_count_diff": float(np.max(np.abs(res["mixed"][2] - res["fp64"][2])
+                                                         / np.maximum(1.0, np.abs(res["fp64"][2])))),
+    }))
+    ev.finish()
 
 
 def main():
@@ -192,12 +285,22 @@ def main():
     p.add_argument("--filter-only", action="store_true",
                    help="only time the fused filter kernels and a 512 MiB device copy "
                         "(for hardware-counter runs)")
+    p.add_argument("--kpm-batch", action="store_true",
+                   help="time the batched KPM form instead: DOS requests over a pool of "
+                        "matrices of uneven size (not the headline line)")
+    p.add_argument("--kb-requests", type=int, default=256)
+    p.add_argument("--kb-matrices", type=int, default=32)
+    p.add_argument("--kb-log2n", type=float, nargs=2, default=[14.0, 20.0],
+                   help="matrix sizes log-uniform in [2^lo, 2^hi]")
     args = p.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
                                    "only (slices shard with no communication)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.kpm_batch:
+        kpm_batch_bench(args)
+        return
 
     from evsl_amd.evsl import EVSL, PREC_FP64, PREC_MIXED, VEC_RADEMACHER
     log(f"[bench] building matrix nx={args.nx}")
