@@ -628,6 +628,349 @@ int refine_extract(int n, int kdim, const double* dT, const double* eT, const do
   return 0;
 }
 
+// ---- LanczosGroup ----------------------------------------------------------
+// Slices of one matrix (shared lmin, lmax) advance through the recurrence of
+// ChebLanNr (SRC/cheblanNr.c:150-310) together: every step filters the
+// current Lanczos vector of each active slice in one blocked pass, then runs
+// each slice's recurrence, DGKS re-orthogonalisation and (at its own test
+// points) the convergence test.  A slice added between steps starts with the
+// next step: its start vector is filtered with the other new slices' and
+// normalised, then it takes its first Lanczos step.  Each slice counts its
+// own steps, so its arithmetic (and result) does not depend on which other
+// slices share the group or when they joined.
+struct LanczosGroup::Impl {
+  struct Slot {
+    Slice L;
+    int maxit = 0, k = 0;
+    double tol = 0;
+    const double* v0 = nullptr;
+    SliceResult* out = nullptr;
+    int state = 0;  // 0 free, 1 start pending, 2 active
+  };
+  bool mixed = false;
+  FILE* fstats = nullptr;
+  std::vector<std::unique_ptr<Slot>> slots;
+  DevBuf gt_part, hbuf, scal_d, vrand, parts;
+  int kcap = 0, scap = 0;
+  BlockWork bw;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  size_t part_stride() const {
+    const int n = ctx().n;
+    return std::max((size_t)evsl_dev::kRedBlocks, (size_t)(n + evsl_dev::kWG - 1) / evsl_dev::kWG);
+  }
+  double* part_of(int s) { return parts.as<double>() + (size_t)s * part_stride(); }
+  int ensure(int maxit) {
+    const int n = ctx().n;
+    const int ns = (int)slots.size();
+    if (maxit > kcap) {
+      const int nch = evsl_dev::gemv_t_chunks(n);
+      EVSL_HIP(gt_part.alloc((size_t)nch * (maxit + 1) * sizeof(double)));
+      EVSL_HIP(hbuf.alloc((size_t)(maxit + 1) * sizeof(double)));
+      kcap = maxit;
+    }
+    if (ns > scap) {
+      EVSL_HIP(scal_d.alloc((size_t)4 * ns * sizeof(double)));
+      EVSL_HIP(parts.alloc(part_stride() * ns * sizeof(double)));
+      scap = ns;
+    }
+    EVSL_HIP(vrand.alloc((size_t)n * sizeof(double)));
+    return 0;
+  }
+};
+
+LanczosGroup::LanczosGroup(bool mixed, FILE* fstats) : p_(new Impl) {
+  p_->mixed = mixed;
+  p_->fstats = fstats;
+}
+
+LanczosGroup::~LanczosGroup() {
+  for (auto& e : p_->ev)
+    if (e) (void)hipEventDestroy(e);
+}
+
+int LanczosGroup::active() const {
+  int a = 0;
+  for (auto& s : p_->slots) a += s->state != 0;
+  return a;
+}
+
+int LanczosGroup::add(const double* intv, int maxit, double tol, const double* v0,
+                      const polparams* pol, SliceResult* out) {
+  Impl& g = *p_;
+  Context& c = ctx();
+  *out = SliceResult();
+  if (intv[0] >= intv[1] || intv[0] >= intv[3] || intv[1] <= intv[2]) {
+    out->done = true;  // empty result, as ChebLanNr on an invalid interval
+    out->t_done = evsl_timer();
+    return 0;
+  }
+  if (pol->gam > 1.0 || pol->gam < -1.0) {
+    out->err = -1;
+    out->done = true;
+    out->t_done = evsl_timer();
+    return 0;
+  }
+  int s = -1;
+  for (int i = 0; i < (int)g.slots.size(); ++i)
+    if (g.slots[i]->state == 0) {
+      s = i;
+      break;
+    }
+  if (s < 0) {
+    g.slots.push_back(nullptr);
+    s = (int)g.slots.size() - 1;
+  }
+  g.slots[s] = std::make_unique<Impl::Slot>();  // a fresh slot (a finished one is reused)
+  Impl::Slot& sl = *g.slots[s];
+  std::memcpy(sl.L.intv, intv, 4 * sizeof(double));
+  sl.L.pol = const_cast<polparams*>(pol);
+  sl.maxit = std::min(c.n, maxit);
+  sl.k = 0;
+  sl.tol = tol;
+  sl.v0 = v0;
+  sl.out = out;
+  EVSL_HIP(sl.L.V.alloc((size_t)c.n * (sl.maxit + 1) * sizeof(double)));
+  sl.L.dT.assign(sl.maxit, 0.0);
+  sl.L.eT.assign(sl.maxit, 0.0);
+  sl.state = 1;
+  return g.ensure(sl.maxit);
+}
+
+int LanczosGroup::step() {
+  Impl& g = *p_;
+  Context& c = ctx();
+  const int n = c.n;
+  FILE* fstats = g.fstats;
+  const bool mixed = g.mixed, f32 = mixed;
+  if (!g.ev[0])
+    for (auto& e : g.ev) EVSL_HIP(hipEventCreate(&e));
+  auto elapsed = [](hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess)
+      return 0.0;
+    return ms * 1e-3;
+  };
+  const double eta = 1.0 / std::sqrt(2.0);
+  const int nb_rows = (n + evsl_dev::kWG - 1) / evsl_dev::kWG;
+  double* sc_d = g.scal_d.as<double>();
+  // ---- new slices: filter their start vectors together, normalize ---------
+  {
+    std::vector<int> st;
+    for (int s = 0; s < (int)g.slots.size(); ++s)
+      if (g.slots[s]->state == 1) st.push_back(s);
+    if (!st.empty()) {
+      std::vector<const double*> in;
+      std::vector<double*> out;
+      std::vector<const polparams*> pp;
+      for (int s : st) {
+        in.push_back(g.slots[s]->v0);
+        out.push_back(g.slots[s]->L.vcol(0, n));
+        pp.push_back(g.slots[s]->L.pol);
+      }
+      EVSL_HIP(hipEventRecord(g.ev[0], c.stream));
+      if (int e = filter_columns((int)in.size(), in.data(), out.data(), pp.data(), f32, g.bw))
+        return e;
+      EVSL_HIP(hipEventRecord(g.ev[1], c.stream));
+      c.astats.t_filter += elapsed(g.ev[0], g.ev[1]);
+      for (int s : st) {
+        double* v = g.slots[s]->L.vcol(0, n);
+        EVSL_HIP(evsl_dev::dot_partial(n, v, v, g.part_of(s), c.stream));
+        EVSL_HIP(evsl_dev::sum_scalar(g.part_of(s), evsl_dev::kRedBlocks, sc_d + 4 * s, c.stream));
+      }
+      std::vector<double> nv(4 * g.slots.size());
+      EVSL_HIP(hipMemcpy(nv.data(), sc_d, nv.size() * sizeof(double), hipMemcpyDeviceToHost));
+      for (int s : st) {
+        EVSL_HIP(evsl_dev::scal(n, 1.0 / std::sqrt(nv[4 * s]), g.slots[s]->L.vcol(0, n),
+                                c.stream));
+        g.slots[s]->state = 2;
+        g.slots[s]->v0 = nullptr;
+      }
+    }
+  }
+  std::vector<int> act;
+  for (int s = 0; s < (int)g.slots.size(); ++s)
+    if (g.slots[s]->state == 2) act.push_back(s);
+  if (act.empty()) return 0;
+  const int b = (int)act.size();
+  // ---- one blocked filter pass for all active slices -------------------------
+  {
+    std::vector<const double*> in(b);
+    std::vector<double*> out(b);
+    std::vector<const polparams*> pp(b);
+    for (int j = 0; j < b; ++j) {
+      Impl::Slot& sl = *g.slots[act[j]];
+      in[j] = sl.L.vcol(sl.k, n);
+      out[j] = sl.L.vcol(sl.k + 1, n);
+      pp[j] = sl.L.pol;
+    }
+    EVSL_HIP(hipEventRecord(g.ev[0], c.stream));
+    if (int e = filter_columns(b, in.data(), out.data(), pp.data(), f32, g.bw)) return e;
+    EVSL_HIP(hipEventRecord(g.ev[1], c.stream));
+  }
+  // ---- recurrence: vnew -= beta vold; alpha; vnew -= alpha v; ||vnew||^2 -----
+  for (int s : act) {
+    Impl::Slot& sl = *g.slots[s];
+    Slice& L = sl.L;
+    const int k = sl.k;
+    double* v = L.vcol(k, n);
+    double* vnew = L.vcol(k + 1, n);
+    if (k > 0)
+      EVSL_HIP(evsl_dev::axpy_dot(n, -L.beta, L.vcol(k - 1, n), vnew, v, g.part_of(s), c.stream));
+    else
+      EVSL_HIP(evsl_dev::dot_partial(n, vnew, v, g.part_of(s), c.stream));
+    EVSL_HIP(evsl_dev::sum_scalar(g.part_of(s), evsl_dev::kRedBlocks, sc_d + 4 * s, c.stream));
+    EVSL_HIP(evsl_dev::axpy_dev_nrm(n, -1.0, sc_d + 4 * s, v, vnew, g.part_of(s), c.stream));
+    EVSL_HIP(evsl_dev::sum_scalar(g.part_of(s), evsl_dev::kRedBlocks, sc_d + 4 * s + 1, c.stream));
+    // first Gram-Schmidt pass
+    EVSL_HIP(evsl_dev::gemv_t(n, k + 1, L.V.as<double>(), n, vnew, g.gt_part.as<double>(),
+                              g.hbuf.as<double>(), c.stream));
+    EVSL_HIP(evsl_dev::gemv_n_sub_nrm(n, k + 1, L.V.as<double>(), n, g.hbuf.as<double>(), vnew,
+                                      g.part_of(s), c.stream));
+    EVSL_HIP(evsl_dev::sum_scalar(g.part_of(s), nb_rows, sc_d + 4 * s + 2, c.stream));
+  }
+  std::vector<double> hs(4 * g.slots.size());
+  EVSL_HIP(hipMemcpy(hs.data(), sc_d, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<int> second;
+  for (int s : act) {
+    Impl::Slot& sl = *g.slots[s];
+    Slice& L = sl.L;
+    const int k = sl.k;
+    const double alpha = hs[4 * s], old_nrm = std::sqrt(hs[4 * s + 1]);
+    L.dT[k] = alpha;
+    L.wn += std::fabs(alpha);
+    L.nrm = std::sqrt(hs[4 * s + 2]);
+    c.astats.n_reorth_pass++;
+    c.astats.reorth_bytes += 16.0 * n * (k + 1) + 24.0 * n;
+    if (!(L.nrm > eta * old_nrm)) second.push_back(s);
+  }
+  for (int s : second) {
+    Impl::Slot& sl = *g.slots[s];
+    Slice& L = sl.L;
+    const int k = sl.k;
+    double* vnew = L.vcol(k + 1, n);
+    EVSL_HIP(evsl_dev::gemv_t(n, k + 1, L.V.as<double>(), n, vnew, g.gt_part.as<double>(),
+                              g.hbuf.as<double>(), c.stream));
+    EVSL_HIP(evsl_dev::gemv_n_sub_nrm(n, k + 1, L.V.as<double>(), n, g.hbuf.as<double>(), vnew,
+                                      g.part_of(s), c.stream));
+    EVSL_HIP(evsl_dev::sum_scalar(g.part_of(s), nb_rows, sc_d + 4 * s + 3, c.stream));
+    c.astats.n_reorth_pass++;
+    c.astats.reorth_bytes += 16.0 * n * (k + 1) + 24.0 * n;
+  }
+  if (!second.empty()) {
+    EVSL_HIP(hipMemcpy(hs.data(), sc_d, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int s : second) g.slots[s]->L.nrm = std::sqrt(hs[4 * s + 3]);
+  }
+  for (int s : act) {
+    Impl::Slot& sl = *g.slots[s];
+    Slice& L = sl.L;
+    const int k = sl.k;
+    double* vnew = L.vcol(k + 1, n);
+    L.beta = L.nrm;
+    L.wn += 2.0 * L.beta;
+    L.nwn += 3;
+    if (L.beta * L.nwn < kOrthTol * L.wn) {  // lucky breakdown: new filtered random vector
+      rand_double_device(n, g.vrand.as<double>());
+      ChebWork cw;
+      if (int e = cheb_av(L.pol, g.vrand.as<double>(), vnew, cw, mixed ? EVSL_PREC_MIXED : 0))
+        return e;
+      double d2[2];
+      EVSL_HIP(evsl_dev::dot_partial(n, vnew, vnew, red_part(), c.stream));
+      EVSL_HIP(evsl_dev::sum_scalar(red_part(), evsl_dev::kRedBlocks, red_scalar(0), c.stream));
+      if (int e = read_scalars(red_scalar(0), 1, d2)) return e;
+      double old = std::sqrt(d2[0]), nn = old;
+      for (int it = 0; it < 2; ++it) {
+        EVSL_HIP(evsl_dev::gemv_t(n, k + 1, L.V.as<double>(), n, vnew, g.gt_part.as<double>(),
+                                  g.hbuf.as<double>(), c.stream));
+        EVSL_HIP(evsl_dev::gemv_n_sub_nrm(n, k + 1, L.V.as<double>(), n, g.hbuf.as<double>(),
+                                          vnew, red_part(), c.stream));
+        EVSL_HIP(evsl_dev::sum_scalar(red_part(), nb_rows, red_scalar(1), c.stream));
+        if (int e = read_scalars(red_scalar(1), 1, d2)) return e;
+        nn = std::sqrt(d2[0]);
+        if (nn > eta * old) break;
+        old = nn;
+      }
+      EVSL_HIP(evsl_dev::scal(n, 1.0 / nn, vnew, c.stream));
+      L.beta = 0.0;
+    } else {
+      EVSL_HIP(evsl_dev::scal(n, 1.0 / L.beta, vnew, c.stream));
+    }
+    L.eT[k] = L.beta;
+  }
+  EVSL_HIP(hipEventRecord(g.ev[2], c.stream));
+  c.astats.t_filter += elapsed(g.ev[0], g.ev[1]);
+  c.astats.t_reorth += elapsed(g.ev[1], g.ev[2]);
+  c.astats.n_lanczos += b;
+  // ---- convergence tests (restricted trace, every 20 steps from 30) ---------
+  const int Ntest = 30, cycle = 20;
+  std::vector<int> tst;
+  for (int s : act) {
+    const int k = g.slots[s]->k;
+    if (!((k < Ntest || (k - Ntest) % cycle != 0) && k != g.slots[s]->maxit - 1)) tst.push_back(s);
+  }
+  // the slices' tridiagonal eigenvalue problems are independent: one host
+  // thread each (the device is idle until the next filter pass)
+  const int nt = (int)tst.size();
+  std::vector<double> tr1s(nt, 0.0);
+  std::vector<int> nconvs(nt, 0), qlerr(nt, 0);
+#pragma omp parallel for schedule(dynamic, 1) if (nt > 1)
+  for (int j = 0; j < nt; ++j) {
+    const Impl::Slot& sl = *g.slots[tst[j]];
+    const Slice& L = sl.L;
+    const int kdim = sl.k + 1;
+    std::vector<double> ev(kdim), e(kdim + 1, 0.0);
+    std::memcpy(ev.data(), L.dT.data(), kdim * sizeof(double));
+    std::memcpy(e.data(), L.eT.data(), (kdim - 1) * sizeof(double));
+    if (tridiag_ql(kdim, ev.data(), e.data(), nullptr)) {
+      qlerr[j] = 1;
+      continue;
+    }
+    for (int i = 0; i < kdim; ++i)
+      if (ev[i] + kEpsM >= L.pol->bar) {
+        tr1s[j] += ev[i];
+        ++nconvs[j];
+      }
+  }
+  for (int j = 0; j < nt; ++j) {
+    const int s = tst[j];
+    Impl::Slot& sl = *g.slots[s];
+    Slice& L = sl.L;
+    const int k = sl.k;
+    if (qlerr[j]) return EVSL_ERR_EIG;
+    const double tr1 = tr1s[j];
+    const int nconv = nconvs[j];
+    // FP64: the reference's test.  Mixed: the filtered operator is known to
+    // ~1e-6, so the trace is compared at that level and the count of Ritz
+    // values above bar must also be unchanged.
+    const double dtr = std::fabs(tr1 - L.tr0);
+    bool done = mixed ? (dtr < 1e-7 * std::max(1, nconv) && nconv == L.last_nconv)
+                      : (dtr < 2e-12 || std::fabs(tr1) + std::fabs(L.tr0) < 1e-10);
+    if (fstats)
+      std::fprintf(fstats, "slice %d k %4d: nconv %4d  tr1 %21.15e dtr %.3e\n", s, k, nconv, tr1,
+                   dtr);
+    L.tr0 = tr1;
+    L.last_nconv = nconv;
+    if (done || k == sl.maxit - 1) {
+      L.kdim = k + 1;
+      const double tr = evsl_timer();
+      if (fstats) std::fprintf(fstats, "slice %d done at k %d\n", s, k);
+      SliceResult* o = sl.out;
+      int err = refine_extract(n, L.kdim, L.dT.data(), L.eT.data(), L.V.as<double>(), L.pol,
+                               L.intv, sl.tol, mixed, &o->nev, &o->lam, &o->W, &o->res, fstats);
+      c.astats.t_ritz += evsl_timer() - tr;
+      L.V.release();
+      o->err = err;
+      o->steps = L.kdim;
+      o->done = true;
+      o->t_done = evsl_timer();
+      sl.state = 0;
+      if (err) return err;
+    }
+  }
+  for (int s : act)
+    if (g.slots[s]->state == 2) ++g.slots[s]->k;
+  return 0;
+}
+
 }  // namespace evsl
 
 using namespace evsl;
@@ -641,11 +984,8 @@ int ChebLanNr_batch(int nslices, double* intvs, int maxit, double tol, double* v
   Context& c = ctx();
   if (c.op == 0) return EVSL_ERR_NOMAT;
   if (nslices <= 0) return EVSL_ERR_ARG;
-  const int n = c.n;
-  const size_t nl = n;
-  maxit = std::min(n, maxit);
+  const size_t nl = c.n;
   const bool mixed = (c.filter_prec == EVSL_PREC_MIXED) && c.op == 1;
-  const bool f32 = mixed;
   const double cc0 = pols[0].cc, dd0 = pols[0].dd;
   for (int s = 0; s < nslices; ++s) {
     nevOut[s] = 0;
@@ -656,256 +996,24 @@ int ChebLanNr_batch(int nslices, double* intvs, int maxit, double tol, double* v
     }
     if (pols[s].gam > 1.0 || pols[s].gam < -1.0) return -1;
   }
-  // start vector on the device
   DevBuf vtmp;
   const double* v0;
-  {
-    hipPointerAttribute_t at;
-    hipError_t e = hipPointerGetAttributes(&at, vinit);
-    if (e == hipSuccess && (at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged)) {
-      v0 = vinit;
-    } else {
-      (void)hipGetLastError();
-      EVSL_HIP(vtmp.alloc(nl * sizeof(double)));
-      EVSL_HIP(hipMemcpy(vtmp.p, vinit, nl * sizeof(double), hipMemcpyHostToDevice));
-      v0 = vtmp.as<double>();
-    }
-  }
-  std::vector<std::unique_ptr<Slice>> S;
+  if (int e = as_device_vector(vinit, nl, vtmp, &v0)) return e;
+  LanczosGroup grp(mixed, fstats);
+  std::vector<SliceResult> out(nslices);
+  for (int s = 0; s < nslices; ++s)
+    if (int e = grp.add(intvs + 4 * s, maxit, tol, v0, &pols[s], &out[s])) return e;
+  int err = 0;
+  while (grp.active() > 0)
+    if ((err = grp.step())) break;
   for (int s = 0; s < nslices; ++s) {
-    auto sl = std::make_unique<Slice>();
-    std::memcpy(sl->intv, intvs + 4 * s, 4 * sizeof(double));
-    sl->pol = &pols[s];
-    const double* iv = sl->intv;
-    if (iv[0] >= iv[1] || iv[0] >= iv[3] || iv[1] <= iv[2]) sl->active = false;
-    if (sl->active) {
-      EVSL_HIP(sl->V.alloc(nl * (maxit + 1) * sizeof(double)));
-      sl->dT.resize(maxit);
-      sl->eT.resize(maxit);
-    }
-    S.push_back(std::move(sl));
+    nevOut[s] = out[s].nev;
+    lamo[s] = out[s].lam;
+    Wo[s] = out[s].W;
+    reso[s] = out[s].res;
+    if (!err && out[s].err) err = out[s].err;
   }
-  const int nch = evsl_dev::gemv_t_chunks(n);
-  DevBuf gt_part, hbuf, scal_d, vrand;
-  EVSL_HIP(gt_part.alloc((size_t)nch * (maxit + 1) * sizeof(double)));
-  EVSL_HIP(hbuf.alloc((size_t)(maxit + 1) * sizeof(double)));
-  EVSL_HIP(scal_d.alloc((size_t)4 * nslices * sizeof(double)));
-  EVSL_HIP(vrand.alloc(nl * sizeof(double)));
-  double* sc_d = scal_d.as<double>();
-  BlockWork bw;
-  const double eta = 1.0 / std::sqrt(2.0);
-  const int nb_rows = (n + evsl_dev::kWG - 1) / evsl_dev::kWG;
-  // partial buffers for the per-slice reductions (one kRedBlocks slot per slice)
-  DevBuf parts;
-  EVSL_HIP(parts.alloc((size_t)std::max(evsl_dev::kRedBlocks, nb_rows) * nslices * sizeof(double)));
-  auto part_of = [&](int s) {
-    return parts.as<double>() + (size_t)s * std::max(evsl_dev::kRedBlocks, nb_rows);
-  };
-  hipEvent_t ev0, ev1, ev2;
-  EVSL_HIP(hipEventCreate(&ev0));
-  EVSL_HIP(hipEventCreate(&ev1));
-  EVSL_HIP(hipEventCreate(&ev2));
-  struct EvGuard {
-    hipEvent_t a, b, cc;
-    ~EvGuard() {
-      (void)hipEventDestroy(a);
-      (void)hipEventDestroy(b);
-      (void)hipEventDestroy(cc);
-    }
-  } evg{ev0, ev1, ev2};
-  auto elapsed = [](hipEvent_t a, hipEvent_t b) {
-    float ms = 0.f;
-    if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess)
-      return 0.0;
-    return ms * 1e-3;
-  };
-  // ---- filter the start vector for every slice, normalize -------------------
-  {
-    std::vector<const double*> in;
-    std::vector<double*> out;
-    std::vector<const polparams*> pp;
-    for (auto& sl : S)
-      if (sl->active) {
-        in.push_back(v0);
-        out.push_back(sl->vcol(0, n));
-        pp.push_back(sl->pol);
-      }
-    EVSL_HIP(hipEventRecord(ev0, c.stream));
-    if (int e = filter_columns((int)in.size(), in.data(), out.data(), pp.data(), f32, bw)) return e;
-    EVSL_HIP(hipEventRecord(ev1, c.stream));
-    c.astats.t_filter += elapsed(ev0, ev1);
-    for (int s = 0; s < nslices; ++s) {
-      if (!S[s]->active) continue;
-      EVSL_HIP(evsl_dev::dot_partial(n, S[s]->vcol(0, n), S[s]->vcol(0, n), part_of(s), c.stream));
-      EVSL_HIP(evsl_dev::sum_scalar(part_of(s), evsl_dev::kRedBlocks, sc_d + s, c.stream));
-    }
-    std::vector<double> nv(nslices);
-    EVSL_HIP(hipMemcpy(nv.data(), sc_d, nslices * sizeof(double), hipMemcpyDeviceToHost));
-    for (int s = 0; s < nslices; ++s)
-      if (S[s]->active)
-        EVSL_HIP(evsl_dev::scal(n, 1.0 / std::sqrt(nv[s]), S[s]->vcol(0, n), c.stream));
-  }
-  const int Ntest = 30, cycle = 20;
-  std::vector<double> hs(4 * nslices);
-  for (int k = 0; k < maxit; ++k) {
-    std::vector<int> act;
-    for (int s = 0; s < nslices; ++s)
-      if (S[s]->active) act.push_back(s);
-    if (act.empty()) break;
-    const int b = (int)act.size();
-    // ---- one blocked filter pass for all active slices ----------------------
-    {
-      std::vector<const double*> in(b);
-      std::vector<double*> out(b);
-      std::vector<const polparams*> pp(b);
-      for (int j = 0; j < b; ++j) {
-        in[j] = S[act[j]]->vcol(k, n);
-        out[j] = S[act[j]]->vcol(k + 1, n);
-        pp[j] = S[act[j]]->pol;
-      }
-      EVSL_HIP(hipEventRecord(ev0, c.stream));
-      if (int e = filter_columns(b, in.data(), out.data(), pp.data(), f32, bw)) return e;
-      EVSL_HIP(hipEventRecord(ev1, c.stream));
-    }
-    // ---- recurrence: vnew -= beta vold; alpha; vnew -= alpha v; ||vnew||^2 ---
-    for (int s : act) {
-      Slice& L = *S[s];
-      double* v = L.vcol(k, n);
-      double* vnew = L.vcol(k + 1, n);
-      if (k > 0)
-        EVSL_HIP(evsl_dev::axpy_dot(n, -L.beta, L.vcol(k - 1, n), vnew, v, part_of(s), c.stream));
-      else
-        EVSL_HIP(evsl_dev::dot_partial(n, vnew, v, part_of(s), c.stream));
-      EVSL_HIP(evsl_dev::sum_scalar(part_of(s), evsl_dev::kRedBlocks, sc_d + 4 * s, c.stream));
-      EVSL_HIP(evsl_dev::axpy_dev_nrm(n, -1.0, sc_d + 4 * s, v, vnew, part_of(s), c.stream));
-      EVSL_HIP(evsl_dev::sum_scalar(part_of(s), evsl_dev::kRedBlocks, sc_d + 4 * s + 1, c.stream));
-      // first Gram-Schmidt pass
-      EVSL_HIP(evsl_dev::gemv_t(n, k + 1, L.V.as<double>(), n, vnew, gt_part.as<double>(),
-                                hbuf.as<double>(), c.stream));
-      EVSL_HIP(evsl_dev::gemv_n_sub_nrm(n, k + 1, L.V.as<double>(), n, hbuf.as<double>(), vnew,
-                                        part_of(s), c.stream));
-      EVSL_HIP(evsl_dev::sum_scalar(part_of(s), nb_rows, sc_d + 4 * s + 2, c.stream));
-    }
-    EVSL_HIP(hipMemcpy(hs.data(), sc_d, 4 * nslices * sizeof(double), hipMemcpyDeviceToHost));
-    std::vector<int> second;
-    for (int s : act) {
-      Slice& L = *S[s];
-      const double alpha = hs[4 * s], old_nrm = std::sqrt(hs[4 * s + 1]);
-      L.dT[k] = alpha;
-      L.wn += std::fabs(alpha);
-      L.nrm = std::sqrt(hs[4 * s + 2]);
-      c.astats.n_reorth_pass++;
-      c.astats.reorth_bytes += 16.0 * n * (k + 1) + 24.0 * n;
-      if (!(L.nrm > eta * old_nrm)) second.push_back(s);
-    }
-    for (int s : second) {
-      Slice& L = *S[s];
-      double* vnew = L.vcol(k + 1, n);
-      EVSL_HIP(evsl_dev::gemv_t(n, k + 1, L.V.as<double>(), n, vnew, gt_part.as<double>(),
-                                hbuf.as<double>(), c.stream));
-      EVSL_HIP(evsl_dev::gemv_n_sub_nrm(n, k + 1, L.V.as<double>(), n, hbuf.as<double>(), vnew,
-                                        part_of(s), c.stream));
-      EVSL_HIP(evsl_dev::sum_scalar(part_of(s), nb_rows, sc_d + 4 * s + 3, c.stream));
-      c.astats.n_reorth_pass++;
-      c.astats.reorth_bytes += 16.0 * n * (k + 1) + 24.0 * n;
-    }
-    if (!second.empty()) {
-      EVSL_HIP(hipMemcpy(hs.data(), sc_d, 4 * nslices * sizeof(double), hipMemcpyDeviceToHost));
-      for (int s : second) S[s]->nrm = std::sqrt(hs[4 * s + 3]);
-    }
-    for (int s : act) {
-      Slice& L = *S[s];
-      double* vnew = L.vcol(k + 1, n);
-      L.beta = L.nrm;
-      L.wn += 2.0 * L.beta;
-      L.nwn += 3;
-      if (L.beta * L.nwn < kOrthTol * L.wn) {  // lucky breakdown: new filtered random vector
-        rand_double_device(n, vrand.as<double>());
-        ChebWork cw;
-        if (int e = cheb_av(L.pol, vrand.as<double>(), vnew, cw, mixed ? EVSL_PREC_MIXED : 0))
-          return e;
-        double d2[2];
-        EVSL_HIP(evsl_dev::dot_partial(n, vnew, vnew, red_part(), c.stream));
-        EVSL_HIP(evsl_dev::sum_scalar(red_part(), evsl_dev::kRedBlocks, red_scalar(0), c.stream));
-        if (int e = read_scalars(red_scalar(0), 1, d2)) return e;
-        double old = std::sqrt(d2[0]), nn = old;
-        for (int it = 0; it < 2; ++it) {
-          EVSL_HIP(evsl_dev::gemv_t(n, k + 1, L.V.as<double>(), n, vnew, gt_part.as<double>(),
-                                    hbuf.as<double>(), c.stream));
-          EVSL_HIP(evsl_dev::gemv_n_sub_nrm(n, k + 1, L.V.as<double>(), n, hbuf.as<double>(),
-                                            vnew, red_part(), c.stream));
-          EVSL_HIP(evsl_dev::sum_scalar(red_part(), nb_rows, red_scalar(1), c.stream));
-          if (int e = read_scalars(red_scalar(1), 1, d2)) return e;
-          nn = std::sqrt(d2[0]);
-          if (nn > eta * old) break;
-          old = nn;
-        }
-        EVSL_HIP(evsl_dev::scal(n, 1.0 / nn, vnew, c.stream));
-        L.beta = 0.0;
-      } else {
-        EVSL_HIP(evsl_dev::scal(n, 1.0 / L.beta, vnew, c.stream));
-      }
-      L.eT[k] = L.beta;
-    }
-    EVSL_HIP(hipEventRecord(ev2, c.stream));
-    c.astats.t_filter += elapsed(ev0, ev1);
-    c.astats.t_reorth += elapsed(ev1, ev2);
-    c.astats.n_lanczos += b;
-    // ---- convergence tests (restricted trace, every 20 steps from 30) --------
-    if ((k < Ntest || (k - Ntest) % cycle != 0) && k != maxit - 1) continue;
-    // the slices' tridiagonal eigenvalue problems are independent: one host
-    // thread each (the device is idle until the next filter pass)
-    const int kdim = k + 1;
-    std::vector<double> tr1s(b, 0.0);
-    std::vector<int> nconvs(b, 0), qlerr(b, 0);
-#pragma omp parallel for schedule(dynamic, 1) if (b > 1)
-    for (int j = 0; j < b; ++j) {
-      const Slice& L = *S[act[j]];
-      std::vector<double> ev(kdim), e(kdim + 1, 0.0);
-      std::memcpy(ev.data(), L.dT.data(), kdim * sizeof(double));
-      std::memcpy(e.data(), L.eT.data(), (kdim - 1) * sizeof(double));
-      if (tridiag_ql(kdim, ev.data(), e.data(), nullptr)) {
-        qlerr[j] = 1;
-        continue;
-      }
-      for (int i = 0; i < kdim; ++i)
-        if (ev[i] + kEpsM >= L.pol->bar) {
-          tr1s[j] += ev[i];
-          ++nconvs[j];
-        }
-    }
-    for (int j = 0; j < b; ++j) {
-      const int s = act[j];
-      Slice& L = *S[s];
-      if (qlerr[j]) return EVSL_ERR_EIG;
-      const double tr1 = tr1s[j];
-      const int nconv = nconvs[j];
-      // FP64: the reference's test.  Mixed: the filtered operator is known to
-      // ~1e-6, so the trace is compared at that level and the count of Ritz
-      // values above bar must also be unchanged.
-      const double dtr = std::fabs(tr1 - L.tr0);
-      bool done = mixed ? (dtr < 1e-7 * std::max(1, nconv) && nconv == L.last_nconv)
-                        : (dtr < 2e-12 || std::fabs(tr1) + std::fabs(L.tr0) < 1e-10);
-      if (fstats)
-        std::fprintf(fstats, "slice %d k %4d: nconv %4d  tr1 %21.15e dtr %.3e\n", s, k, nconv,
-                     tr1, dtr);
-      L.tr0 = tr1;
-      L.last_nconv = nconv;
-      if (done || k == maxit - 1) {
-        L.kdim = kdim;
-        L.active = false;
-        const double tr = evsl_timer();
-        if (fstats) std::fprintf(fstats, "slice %d done at k %d\n", s, k);
-        int err = refine_extract(n, kdim, L.dT.data(), L.eT.data(), L.V.as<double>(), L.pol,
-                                 L.intv, tol, mixed, &nevOut[s], &lamo[s], &Wo[s], &reso[s],
-                                 fstats);
-        c.astats.t_ritz += evsl_timer() - tr;
-        L.V.release();
-        if (err) return err;
-      }
-    }
-  }
-  return 0;
+  return err;
 }
 
 

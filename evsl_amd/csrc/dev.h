@@ -97,7 +97,8 @@ bool cheb_spmm_streaming(const Csr& A, int B, bool fp32, Mode mode);
 // layout ((row, ia[row]) pairs), the start block v0 and the ring of three
 // T_k blocks (n x B interleaved, element type of the launch), the block
 // partials of the moment dots, and the recurrence
-// T_{k+1} = t (A - cc I) T_k - T_{k-1} with factor t0 at k = 0.
+// T_{k+1} = t (A - cc I) T_k - T_{k-1} with factor t0 at k = 0.  A launch
+// at step k advances this matrix's recurrence at its own step k - k0.
 struct KpmGroupEntry {
   const int* ia;
   const int* ja;
@@ -108,6 +109,7 @@ struct KpmGroupEntry {
   double* part;
   double t0, t, cc;
   int n, blk0;  // rows; first global row block of this matrix
+  int k0, pad;
 };
 // The partial sums of one grouped matrix: nblk block partials (B per block)
 // summed as sum_block_partials does (G first-level blocks into scratch, then
@@ -117,6 +119,7 @@ struct KpmSumEntry {
   double* scratch;
   double* dots;
   int nblk, G;
+  int k0, pad;
 };
 // One grouped KPM step k over `total` row blocks (owner[gb] = entry of row
 // block gb) at block width B (lane layout of the kKpm policy), then its

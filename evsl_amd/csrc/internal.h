@@ -5,6 +5,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "../../include/evsl.h"
@@ -68,6 +69,12 @@ struct DevMatrix {
 // Build the LDS row-block partition (at most rows_cap rows and kNnzCap
 // nonzeros per block) from host row pointers.
 std::vector<int> make_row_blocks(const int* ia_h, int n, int rows_cap);
+// A device operator M from a device CSR: row-block tables of every width,
+// 16-bit column offsets when they fit, the stored-diagonal flag; *red_need =
+// reduction scratch (doubles) its launches need (see ensure_red_part)
+int build_device_matrix(int n, int nnz, int* ia_d, int* ja_d, double* a_d, const int* ia_h,
+                        const int* ja_h, bool owns, DevMatrix& M, size_t* red_need);
+int ensure_red_part(size_t need);
 // make sure the shifted FP32 copy for (cc, scale) exists; false if the
 // matrix lacks stored diagonal entries
 int ensure_a32_shifted(double cc, double scale, bool* ok);
@@ -169,6 +176,37 @@ int check_intv(const double* intv);
 // device pointer for n doubles the caller holds on the host or the device
 // (a host array is copied into tmp)
 int as_device_vector(const double* p, size_t n, DevBuf& tmp, const double** out);
+// Outputs of one slice of a LanczosGroup, with ChebLanNr's ownership: lam,
+// res host (evsl_Free), W device (evsl_Free_device)
+struct SliceResult {
+  int nev = 0;
+  double* lam = nullptr;
+  double* W = nullptr;
+  double* res = nullptr;
+  int err = 0;
+  int steps = 0;
+  bool done = false;
+  double t_done = 0.0;
+};
+// Slices of the current matrix (shared lmin, lmax) advancing through the
+// filtered Lanczos recurrence together, one step at a time; a slice added
+// between steps joins with the next one.  pol and v0 must stay valid until
+// the slice is done (v0 until its first step).
+class LanczosGroup {
+ public:
+  LanczosGroup(bool mixed, FILE* fstats);
+  ~LanczosGroup();
+  LanczosGroup(const LanczosGroup&) = delete;
+  LanczosGroup& operator=(const LanczosGroup&) = delete;
+  int add(const double* intv, int maxit, double tol, const double* v0, const polparams* pol,
+          SliceResult* out);
+  int step();
+  int active() const;
+
+ private:
+  struct Impl;
+  std::unique_ptr<Impl> p_;
+};
 // KPM moments and eigenvalue count from raw[k] = sum over the nvec vectors of
 // v^T T_k(A~) v (k >= 1), as kpmdos finishes them
 void kpm_moments(int Mdeg, int damping, int nvec, const double* intv, int n, const double* raw,

@@ -351,6 +351,7 @@ __global__ __launch_bounds__(kWG) void k_kpm_group(const KpmGroupEntry* __restri
                                                    int k) {
   const int gb = xcd_remap(blockIdx.x, total);
   const KpmGroupEntry& e = tab[owner[gb]];
+  k -= e.k0;  // this matrix's own step
   const size_t nb = (size_t)e.n * B;
   const VT* x0 = static_cast<const VT*>(e.x0);
   VT* ring = static_cast<VT*>(e.ring);
@@ -404,7 +405,7 @@ __global__ __launch_bounds__(kWG) void k_kpm_group_sums(const KpmSumEntry* __res
   __shared__ double s_red[kWG];
   const int r = owner[blockIdx.x];
   const KpmSumEntry& e = tab[r];
-  double* out = e.dots + (size_t)(k + 1) * B;
+  double* out = e.dots + (size_t)(k - e.k0 + 1) * B;
   if (level == 1) {
     if (e.G == 1)
       sum_partials_block(e.part, e.nblk, B, out, 0, 1, s_red);

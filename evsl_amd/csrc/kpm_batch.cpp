@@ -220,6 +220,8 @@ int kpmdos_batch(int nreq, csrMat* const* mats, const double* intvs, int Mdeg, i
       e.cc = q.cc;
       e.n = m.n;
       e.blk0 = total;
+      e.k0 = 0;
+      e.pad = 0;
       owner.insert(owner.end(), nblk, r);
       total += nblk;
       evsl_dev::KpmSumEntry& s = sums[r];
@@ -228,6 +230,8 @@ int kpmdos_batch(int nreq, csrMat* const* mats, const double* intvs, int Mdeg, i
       s.dots = q.dots.as<double>();
       s.nblk = nblk;
       s.G = G;
+      s.k0 = 0;
+      s.pad = 0;
       first1[r] = (int)owner1.size();
       owner1.insert(owner1.end(), G, r);
       if (G > 1) owner2.push_back(r);

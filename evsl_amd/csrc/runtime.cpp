@@ -151,18 +151,19 @@ int apply_A(const double* x, double* y) {
   return EVSL_ERR_NOMAT;
 }
 
-// register a device CSR (arrays already on the device); builds the row-block
-// tables of every block width and checks for stored diagonal entries
-static int register_device_csr(int n, int nnz, int* ia_d, int* ja_d, double* a_d,
-                               const int* ia_h, const int* ja_h, bool owns) {
-  Context& c = g_ctx;
-  c.A.release();
-  c.A.n = n;
-  c.A.nnz = nnz;
-  c.A.ia = ia_d;
-  c.A.ja = ja_d;
-  c.A.a64 = a_d;
-  c.A.owns_csr = owns;
+// A device operator from a device CSR (arrays already on the device): the
+// row-block tables of every block width, 16-bit column offsets when they fit,
+// and whether every row stores its diagonal; *red_need = the reduction
+// scratch (doubles) its launches need
+int build_device_matrix(int n, int nnz, int* ia_d, int* ja_d, double* a_d, const int* ia_h,
+                        const int* ja_h, bool owns, DevMatrix& M, size_t* red_need) {
+  M.release();
+  M.n = n;
+  M.nnz = nnz;
+  M.ia = ia_d;
+  M.ja = ja_d;
+  M.a64 = a_d;
+  M.owns_csr = owns;
   std::vector<int> ia_tmp, ja_tmp;
   if (!ia_h) {
     ia_tmp.resize(n + 1);
@@ -184,7 +185,7 @@ static int register_device_csr(int n, int nnz, int* ia_d, int* ja_d, double* a_d
       }
     full = has;
   }
-  c.A.full_diag = full;
+  M.full_diag = full;
   // 16-bit column offsets when every |ja - row| fits (banded and stencil
   // matrices in a local ordering): 2 bytes less per nonzero and filter step
   {
@@ -201,31 +202,52 @@ static int register_device_csr(int n, int nnz, int* ia_d, int* ja_d, double* a_d
       std::vector<short> jd((size_t)nnz);
       for (int r = 0; r < n; ++r)
         for (int k = ia_h[r]; k < ia_h[r + 1]; ++k) jd[k] = (short)(ja_h[k] - r);
-      EVSL_HIP(hipMalloc((void**)&c.A.jd, (size_t)nnz * sizeof(short)));
-      EVSL_HIP(hipMemcpy(c.A.jd, jd.data(), (size_t)nnz * sizeof(short), hipMemcpyHostToDevice));
+      EVSL_HIP(hipMalloc((void**)&M.jd, (size_t)nnz * sizeof(short)));
+      EVSL_HIP(hipMemcpy(M.jd, jd.data(), (size_t)nnz * sizeof(short), hipMemcpyHostToDevice));
     }
   }
   size_t need = (size_t)evsl_dev::kRedBlocks * 64;
   for (int w = 0; w < evsl_dev::kNumWidths; ++w) {
     const int B = evsl_dev::width_of_slot(w);
     std::vector<int> rb = make_row_blocks(ia_h, n, evsl_dev::rows_cap_for(B));
-    c.A.nblk[w] = (int)rb.size() - 1;
+    M.nblk[w] = (int)rb.size() - 1;
     std::vector<int> rbp(2 * rb.size());  // (row, ia[row]) pairs
     for (size_t i = 0; i < rb.size(); ++i) {
       rbp[2 * i] = rb[i];
       rbp[2 * i + 1] = ia_h[rb[i]];
     }
-    EVSL_HIP(hipMalloc((void**)&c.A.rb[w], rbp.size() * sizeof(int)));
-    EVSL_HIP(hipMemcpy(c.A.rb[w], rbp.data(), rbp.size() * sizeof(int), hipMemcpyHostToDevice));
-    need = std::max(need, (size_t)(c.A.nblk[w] + 1) * B);
+    EVSL_HIP(hipMalloc((void**)&M.rb[w], rbp.size() * sizeof(int)));
+    EVSL_HIP(hipMemcpy(M.rb[w], rbp.data(), rbp.size() * sizeof(int), hipMemcpyHostToDevice));
+    need = std::max(need, (size_t)(M.nblk[w] + 1) * B);
+  }
+  // reduction scratch: the largest per-row-block partial buffer, the
+  // first-level sums behind it, and one partial per 256 rows
+  *red_need = std::max(need, (size_t)n / evsl_dev::kWG + 1);
+  return 0;
+}
+
+// the context's reduction scratch holds at least `need` doubles (plus the
+// first-level sums behind them)
+int ensure_red_part(size_t need) {
+  Context& c = g_ctx;
+  const size_t bytes = (need + evsl_dev::kSumScratch) * sizeof(double);
+  if (c.red_part.bytes >= bytes) return 0;
+  EVSL_HIP(c.red_part.alloc(bytes));
+  return 0;
+}
+
+static int register_device_csr(int n, int nnz, int* ia_d, int* ja_d, double* a_d,
+                               const int* ia_h, const int* ja_h, bool owns) {
+  Context& c = g_ctx;
+  size_t need = 0;
+  if (int e = build_device_matrix(n, nnz, ia_d, ja_d, a_d, ia_h, ja_h, owns, c.A, &need)) {
+    c.n = 0;
+    c.op = 0;
+    return e;
   }
   c.n = n;
   c.op = 1;
-  // reduction scratch: the largest per-row-block partial buffer, the
-  // first-level sums behind it, and one partial per 256 rows
-  need = std::max(need, (size_t)n / evsl_dev::kWG + 1);
-  EVSL_HIP(c.red_part.alloc((need + evsl_dev::kSumScratch) * sizeof(double)));
-  return 0;
+  return ensure_red_part(need);
 }
 
 int ensure_a32_shifted(double cc, double scale, bool* ok) {

@@ -43,6 +43,10 @@ EXPORTS = [
     "ChebLanNr_batch", "evsl_amd_read_coo_mm", "evsl_amd_bench_copy",
     "evsl_amd_spmm_vec", "evsl_amd_spmm_index_bytes", "evsl_amd_cheb_av_cols",
     "evsl_amd_spmm_streaming", "kpmdos_batch",
+    "evsl_amd_server_create", "evsl_amd_server_destroy", "evsl_amd_server_add_matrix",
+    "evsl_amd_server_submit_slice", "evsl_amd_server_submit_dos", "evsl_amd_server_step",
+    "evsl_amd_server_status", "evsl_amd_server_slice_result", "evsl_amd_server_dos_result",
+    "evsl_amd_server_stats",
 ]
 
 
@@ -126,6 +130,17 @@ class EVSL:
             "evsl_amd_spmm_streaming": (I, [I, I, I]),
             "kpmdos_batch": (I, [I, C.POINTER(C.POINTER(CsrMat)), dp, I, I, I, I,
                                  C.POINTER(C.c_ulonglong), I, dp, dp]),
+            "evsl_amd_server_create": (VP, [C.c_size_t, I]),
+            "evsl_amd_server_destroy": (V, [VP]),
+            "evsl_amd_server_add_matrix": (I, [VP, C.POINTER(CsrMat), D, D]),
+            "evsl_amd_server_submit_slice": (I, [VP, I, D, D, I, D, pp, dp]),
+            "evsl_amd_server_submit_dos": (I, [VP, I, D, D, I, I, I, I, C.c_ulonglong]),
+            "evsl_amd_server_step": (I, [VP]),
+            "evsl_amd_server_status": (I, [VP, I, dp, dp, dp]),
+            "evsl_amd_server_slice_result": (I, [VP, I, ip, ddp, ddp, ddp]),
+            "evsl_amd_server_dos_result": (I, [VP, I, dp, dp]),
+            "evsl_amd_server_stats": (I, [VP, ip, ip, C.POINTER(C.c_size_t),
+                                          C.POINTER(C.c_size_t)]),
             "evsl_amd_cheb_av_cols": (I, [I, pp, dp, dp, I]),
             "ChebLanNr_batch": (I, [I, dp, I, D, dp, pp, ip, ddp, ddp, ddp, VP]),
         }
@@ -403,3 +418,81 @@ class EVSL:
         nt = "true" if self.lib.evsl_amd_spmm_streaming(B, prec, mode) else "false"
         return (f"k_cheb_spmm<{vt}, {B}, {self.spmm_vec(B, prec, mode)}, {mode}, true, {i16}, "
                 f"{nt}>")
+
+
+class Server:
+    """The request loop of include/evsl_amd.h (evsl_amd_server_*), over an
+    EVSL instance whose context is started."""
+
+    def __init__(self, ev, mem_budget, prec=PREC_FP64):
+        self.ev = ev
+        self.lib = ev.lib
+        self.h = self.lib.evsl_amd_server_create(int(mem_budget), prec)
+        if not self.h:
+            raise RuntimeError(ev.last_error())
+        self._keep = []   # host CSR arrays of registered matrices
+        self.n = {}
+        self.mdeg = {}
+
+    def close(self):
+        if self.h:
+            self.lib.evsl_amd_server_destroy(self.h)
+            self.h = None
+
+    def add_matrix(self, A, lmin, lmax):
+        self._keep.append(A)
+        mid = self.lib.evsl_amd_server_add_matrix(self.h, C.byref(A.csr), lmin, lmax)
+        if mid >= 0:
+            self.n[mid] = A.n
+        return mid
+
+    def submit_slice(self, mat, a, b, maxit, tol, vinit, pol=None):
+        v = np.ascontiguousarray(vinit, dtype=np.float64)
+        return self.lib.evsl_amd_server_submit_slice(self.h, mat, a, b, maxit, tol,
+                                                     C.byref(pol) if pol is not None else None,
+                                                     dptr(v))
+
+    def submit_dos(self, mat, a, b, Mdeg, damping, nvec, vec_kind=VEC_RADEMACHER, seed=0):
+        rid = self.lib.evsl_amd_server_submit_dos(self.h, mat, a, b, Mdeg, damping, nvec,
+                                                  vec_kind, seed)
+        if rid >= 0:
+            self.mdeg[rid] = Mdeg
+        return rid
+
+    def step(self):
+        return self.lib.evsl_amd_server_step(self.h)
+
+    def status(self, rid):
+        ts, t0, td = C.c_double(), C.c_double(), C.c_double()
+        st = self.lib.evsl_amd_server_status(self.h, rid, C.byref(ts), C.byref(t0), C.byref(td))
+        return st, ts.value, t0.value, td.value
+
+    def stats(self):
+        q, r = C.c_int(), C.c_int()
+        m, pk = C.c_size_t(), C.c_size_t()
+        self.lib.evsl_amd_server_stats(self.h, C.byref(q), C.byref(r), C.byref(m), C.byref(pk))
+        return {"queued": q.value, "running": r.value, "mem_in_use": m.value, "mem_peak": pk.value}
+
+    def slice_result(self, rid, n):
+        nev = C.c_int()
+        lam, Y, res = c_double_p(), c_double_p(), c_double_p()
+        err = self.lib.evsl_amd_server_slice_result(self.h, rid, C.byref(nev), C.byref(lam),
+                                                    C.byref(Y), C.byref(res))
+        k = nev.value
+        lam_h = np.array([lam[i] for i in range(k)]) if k else np.zeros(0)
+        res_h = np.array([res[i] for i in range(k)]) if k else np.zeros(0)
+        Yh = np.zeros((n, 0))
+        if k:
+            Yh = self.ev.from_device(C.cast(Y, C.c_void_p).value, n * k).reshape(k, n).T
+            self.lib._evsl_Free_device(C.cast(Y, C.c_void_p))
+        if lam:
+            self.lib._evsl_Free(C.cast(lam, C.c_void_p))
+        if res:
+            self.lib._evsl_Free(C.cast(res, C.c_void_p))
+        return err, lam_h, Yh, res_h
+
+    def dos_result(self, rid):
+        mu = np.zeros(self.mdeg[rid] + 1)
+        ec = C.c_double()
+        err = self.lib.evsl_amd_server_dos_result(self.h, rid, dptr(mu), C.byref(ec))
+        return err, mu, ec.value

@@ -102,6 +102,40 @@ int ChebLanNr_batch(int nslices, double *intvs, int maxit, double tol, double *v
 int evsl_amd_cheb_av_cols(int ncols, const polparams *pols, const double *X, double *Y,
                           int prec);
 
+/* A request loop over device-resident matrices (no counterpart in the
+ * reference; its batched slicing and DOS forms served as requests arrive).
+ * Matrices registered with evsl_amd_server_add_matrix stay in HBM.  Slice
+ * requests (ChebLanNr on [a, b] of a registered matrix with its [lmin, lmax],
+ * Krylov dimension maxit, filter from find_pol with pol_in's inputs or the
+ * drivers' defaults when NULL, start vector vinit host or device) and DOS
+ * requests (kpmdos_ex with seeded device vectors; count over [a, b]) are
+ * queued by submit; each evsl_amd_server_step admits queued requests in
+ * arrival order while their projected memory (n x (maxit+2) x 8 bytes for a
+ * slice) fits mem_budget, advances every running slice of each matrix by one
+ * filtered Lanczos step in one blocked pass, and every running DOS request by
+ * up to 64 Chebyshev degrees in grouped launches.  A request's result does
+ * not depend on the other requests sharing its steps.  step returns the
+ * number of requests not yet done (0 when idle) or an error.  status returns
+ * 0 queued, 1 running, 2 done, with evsl_timer() stamps.  Results: slice ->
+ * lam, res host (evsl_Free), Y device (evsl_Free_device), ownership passes
+ * to the caller; DOS -> mu[Mdeg+1], ecnt.  Single-threaded, like the context. */
+typedef struct evslAmdServer evslAmdServer;
+evslAmdServer *evsl_amd_server_create(size_t mem_budget, int prec);
+void evsl_amd_server_destroy(evslAmdServer *s);
+int evsl_amd_server_add_matrix(evslAmdServer *s, csrMat *A, double lmin, double lmax);
+int evsl_amd_server_submit_slice(evslAmdServer *s, int mat, double a, double b, int maxit,
+                                 double tol, const polparams *pol_in, const double *vinit);
+int evsl_amd_server_submit_dos(evslAmdServer *s, int mat, double a, double b, int Mdeg,
+                               int damping, int nvec, int vec_kind, unsigned long long seed);
+int evsl_amd_server_step(evslAmdServer *s);
+int evsl_amd_server_status(evslAmdServer *s, int req, double *t_submit, double *t_start,
+                           double *t_done);
+int evsl_amd_server_slice_result(evslAmdServer *s, int req, int *nev, double **lam,
+                                 double **Y, double **res);
+int evsl_amd_server_dos_result(evslAmdServer *s, int req, double *mu, double *ecnt);
+int evsl_amd_server_stats(evslAmdServer *s, int *queued, int *running, size_t *mem_in_use,
+                          size_t *mem_peak);
+
 /* Matrix Market coordinate reader of the reference drivers
  * (TESTS/COMMON/io.c:115-217, read_coo_MM): real/integer, general/symmetric
  * (the stored triangle, then its mirrored off-diagonal entries); indices
