@@ -264,6 +264,148 @@ def kpm_batch_bench(args):
     ev.finish()
 
 
+def kpm_count(mu, n, lo, hi, a, b):
+    """Eigenvalue count in [a, b] from normalized KPM moments (the integral
+    kpmdos takes, SRC/spslice.c:120-146)."""
+    c, h = 0.5 * (lo + hi), 0.5 * (hi - lo)
+    t1 = np.arccos(np.clip((a - c) / h, -1 + 1e-15, 1 - 1e-15))
+    t2 = np.arccos(np.clip((b - c) / h, -1 + 1e-15, 1 - 1e-15))
+    k = np.arange(1, len(mu))
+    return n * (mu[0] * (t1 - t2) + np.sum(mu[1:] * (np.sin(k * t1) - np.sin(k * t2)) / k))
+
+
+def serve_bench(args):
+    """Request loop: a Poisson stream (args.sv_rate per second) of
+    args.sv_requests requests -- 70% slices (an interval holding 20-100
+    eigenvalues by the matrix's KPM density, tol 1e-6 or 1e-8, Krylov
+    dimension max(5 (count + 2), 300)), 30% DOS (Mdeg 100-400, 32 Rademacher
+    vectors) -- over args.sv_matrices resident matrices (2-D/3-D Laplacians,
+    random symmetric 8-32 nnz/row, n log-uniform in args.sv_n), served by a
+    mixed-precision and by an FP64 server in turn.  Latency = completion -
+    arrival; eigenpairs/s = accepted pairs / time from first arrival to last
+    completion."""
+    from evsl_amd import matrices as M
+    from evsl_amd.evsl import EVSL, PREC_FP64, PREC_MIXED, VEC_RADEMACHER, Server
+    rng = np.random.default_rng(777)
+    ev = EVSL()
+    assert ev.start() == 0
+    pool = []
+    for i in range(args.sv_matrices):
+        n = int(np.exp(rng.uniform(np.log(args.sv_n[0]), np.log(args.sv_n[1]))))
+        if i % 3 == 0:
+            s2 = int(round(n ** 0.5))
+            A = M.laplacian(s2, s2, 1)
+        elif i % 3 == 1:
+            s3 = int(round(n ** (1 / 3)))
+            A = M.perturb_diag(M.laplacian(s3, s3, s3), 1e-3, seed=i)
+        else:
+            A = M.random_symmetric(n, 8, 32, seed=int(rng.integers(1 << 30)))
+        lo, hi = M.gershgorin(A)
+        assert ev.set_matrix(A) == 0
+        e, mu, _ = ev.kpmdos(300, 1, 40, np.array([lo, hi, lo, hi]), vec_kind=VEC_RADEMACHER,
+                             seed=5)
+        assert e == 0
+        pool.append((A, lo, hi, mu))
+        log(f"[bench] serve: matrix {i}: n {A.n}")
+    reqs = []
+    for r in range(args.sv_requests):
+        m = int(rng.integers(len(pool)))
+        A, lo, hi, mu = pool[m]
+        if rng.uniform() < 0.7:
+            target = int(rng.integers(20, 101))
+            a = lo + (hi - lo) * rng.uniform(0.02, 0.8)
+            bl, bh = a, hi
+            for _ in range(60):  # the b with ~target eigenvalues in [a, b]
+                mid = 0.5 * (bl + bh)
+                if kpm_count(mu, A.n, lo, hi, a, mid) < target:
+                    bl = mid
+                else:
+                    bh = mid
+            maxit = min(A.n, max(5 * (target + 2), 300))
+            reqs.append(("slice", m, a, bh, maxit, float(rng.choice([1e-6, 1e-8])),
+                         int(rng.integers(1 << 30))))
+        else:
+            a = lo + (hi - lo) * rng.uniform(0.0, 0.5)
+            reqs.append(("dos", m, a, a + 0.3 * (hi - lo), int(rng.integers(100, 401)), 32,
+                         int(rng.integers(1 << 30))))
+    arrivals = np.cumsum(rng.exponential(1.0 / args.sv_rate, len(reqs)))
+    out = {}
+    counts = {}
+    for name, prec in (("mixed", PREC_MIXED), ("fp64", PREC_FP64)):
+        srv = Server(ev, 64 << 30, prec)
+        mids = [srv.add_matrix(A, lo, hi) for A, lo, hi, _ in pool]
+        assert min(mids) >= 0, ev.last_error()
+        ids = [None] * len(reqs)
+        sub_t = [0.0] * len(reqs)
+        t0 = time.perf_counter()
+        nxt = 0
+        t_log = t0
+        while True:
+            now = time.perf_counter() - t0
+            if time.perf_counter() - t_log > 15.0:
+                st = srv.stats()
+                log(f"[bench] serve {name}: {now:.0f}s, {nxt} arrived, {st['queued']} queued, "
+                    f"{st['running']} running")
+                t_log = time.perf_counter()
+            while nxt < len(reqs) and arrivals[nxt] <= now:
+                kind, m, a, b, x1, x2, seed = reqs[nxt]
+                if kind == "slice":
+                    v0 = np.random.default_rng(seed).uniform(-1, 1, pool[m][0].n)
+                    ids[nxt] = srv.submit_slice(mids[m], a, b, x1, x2, v0)
+                else:
+                    ids[nxt] = srv.submit_dos(mids[m], a, b, x1, 1, x2, VEC_RADEMACHER, seed)
+                assert ids[nxt] >= 0, ev.last_error()
+                sub_t[nxt] = srv.status(ids[nxt])[1]
+                nxt += 1
+            left = srv.step()
+            assert left >= 0, ev.last_error()
+            if left == 0:
+                if nxt == len(reqs):
+                    break
+                time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
+        lat, pairs, cnt = [], 0, []
+        t_first, t_last = None, None
+        for i, r in enumerate(reqs):
+            st, ts, _, td = srv.status(ids[i])
+            lat.append(td - ts)
+            t_first = ts if t_first is None else min(t_first, ts)
+            t_last = td if t_last is None else max(t_last, td)
+            if r[0] == "slice":
+                err, lam, Y, res = srv.slice_result(ids[i], pool[r[1]][0].n)
+                assert err == 0
+                pairs += len(lam)
+                cnt.append(len(lam))
+        srv.close()
+        span = t_last - t_first
+        lat = np.array(lat)
+        out[name] = {"p50_latency_s": float(np.percentile(lat, 50)),
+                     "p99_latency_s": float(np.percentile(lat, 99)),
+                     "eigenpairs": int(pairs), "span_s": float(span),
+                     "eigenpairs_per_s": pairs / span, "requests_per_s": len(reqs) / span}
+        counts[name] = cnt
+        log(f"[bench] serve {name}: {out[name]}")
+    print(json.dumps({
+        "workload": f"request loop: {len(reqs)} requests (70% slices of 20-100 eigenvalues, tol "
+                    f"1e-6/1e-8; 30% DOS Mdeg 100-400) arriving as a Poisson stream of "
+                    f"{args.sv_rate:g}/s over {len(pool)} resident matrices (2-D/3-D "
+                    f"Laplacians, random symmetric), n in [{args.sv_n[0]:g}, {args.sv_n[1]:g}]",
+        "mixed": out["mixed"], "fp64": out["fp64"],
+        "speedup_eigenpairs_per_s": out["mixed"]["eigenpairs_per_s"] /
+                                    out["fp64"]["eigenpairs_per_s"],
+        "slice_counts_equal_mixed_vs_fp64": counts["mixed"] == counts["fp64"],
+        "slice_count_differences": int(sum(a != b for a, b in zip(counts["mixed"],
+                                                                   counts["fp64"]))),
+        "differing_slices": [
+            {"matrix": r[1], "n": int(pool[r[1]][0].n),
+             "family": ["2-D Laplacian", "3-D Laplacian", "random symmetric"][r[1] % 3],
+             "interval": [float(r[2]), float(r[3])], "maxit": int(r[4]), "tol": r[5],
+             "mixed": cm, "fp64": cf}
+            for r, cm, cf in zip([q for q in reqs if q[0] == "slice"], counts["mixed"],
+                                 counts["fp64"]) if cm != cf],
+    }))
+    ev.finish()
+
+
 def main():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -292,6 +434,13 @@ def main():
     p.add_argument("--kb-matrices", type=int, default=32)
     p.add_argument("--kb-log2n", type=float, nargs=2, default=[14.0, 20.0],
                    help="matrix sizes log-uniform in [2^lo, 2^hi]")
+    p.add_argument("--serve", action="store_true",
+                   help="time the request loop instead: a Poisson stream of slice and DOS "
+                        "requests over a resident matrix pool (not the headline line)")
+    p.add_argument("--sv-requests", type=int, default=32)
+    p.add_argument("--sv-matrices", type=int, default=12)
+    p.add_argument("--sv-rate", type=float, default=200.0, help="mean arrivals per second")
+    p.add_argument("--sv-n", type=float, nargs=2, default=[1e4, 1e5])
     args = p.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
@@ -300,6 +449,9 @@ def main():
         sys.exit(2)
     if args.kpm_batch:
         kpm_batch_bench(args)
+        return
+    if args.serve:
+        serve_bench(args)
         return
 
     from evsl_amd.evsl import EVSL, PREC_FP64, PREC_MIXED, VEC_RADEMACHER

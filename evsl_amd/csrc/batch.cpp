@@ -463,8 +463,10 @@ int residual_filter(int n, int m, double* U, double* W, const polparams* pol, Bl
 // Ritz block, FP64 filter pass(es), Cholesky QR and FP64 Rayleigh-Ritz.
 int refine_extract(int n, int kdim, const double* dT, const double* eT, const double* V,
                    const polparams* pol, const double* intv, double tol, bool mixed,
-                   int* nevOut, double** lamo, double** Wo, double** reso, FILE* fstats) {
+                   int* nevOut, double** lamo, double** Wo, double** reso, FILE* fstats,
+                   bool* stalled) {
   Context& c = ctx();
+  if (stalled) *stalled = false;
   const double bar = pol->bar, aa = intv[0], bb = intv[1];
   std::vector<double> eval(kdim), e(kdim + 1, 0.0);
   std::memcpy(eval.data(), dT, kdim * sizeof(double));
@@ -594,7 +596,13 @@ int refine_extract(int n, int kdim, const double* dT, const double* eT, const do
       }
       if (fstats)
         std::fprintf(fstats, "  refinement pass %d: %d near-converged in [a,b]\n", pass + 1, near);
-      if (near == 0 || pass + 1 >= min_pass + kMaxExtraPasses) break;
+      if (near == 0) break;
+      if (pass + 1 >= min_pass + kMaxExtraPasses) {
+        // the passes no longer close the gap to tol (a filter too weak to
+        // separate the guard set): the caller may redo the slice in FP64
+        if (stalled) *stalled = true;
+        break;
+      }
       ++c.astats.n_refine_extra;
     }
   }
@@ -631,18 +639,29 @@ int refine_extract(int n, int kdim, const double* dT, const double* eT, const do
 // ---- LanczosGroup ----------------------------------------------------------
 // Slices of one matrix (shared lmin, lmax) advance through the recurrence of
 // ChebLanNr (SRC/cheblanNr.c:150-310) together: every step filters the
-// current Lanczos vector of each active slice in one blocked pass, then runs
-// each slice's recurrence, DGKS re-orthogonalisation and (at its own test
-// points) the convergence test.  A slice added between steps starts with the
-// next step: its start vector is filtered with the other new slices' and
-// normalised, then it takes its first Lanczos step.  Each slice counts its
-// own steps, so its arithmetic (and result) does not depend on which other
-// slices share the group or when they joined.
+// current Lanczos vector of each active slice in one blocked pass (one per
+// filter precision), then runs each slice's recurrence, DGKS
+// re-orthogonalisation and (at its own test points) the convergence test.  A
+// slice added between steps starts with the next step: its start vector is
+// filtered with the other new slices' and normalised, then it takes its first
+// Lanczos step.  Each slice counts its own steps, so its arithmetic (and
+// result) does not depend on which other slices share the group or when they
+// joined.
+//
+// In the mixed form a slice filters in FP32 only when its polynomial has a
+// minimum degree: a lower-degree filter carries few of the bytes (the
+// re-orthogonalisation dominates) and separates the refinement's guard set
+// too weakly (an end interval's degree-4 filter left every pair at ~2e-7
+// after three refinement passes); the minimum is set by
+// evsl_amd_set_mixed_min_degree (default 32).  A mixed slice whose refinement still
+// stalls above tol is run again from its start vector in FP64.
+
 struct LanczosGroup::Impl {
   struct Slot {
     Slice L;
     int maxit = 0, k = 0;
     double tol = 0;
+    bool f32 = false;
     const double* v0 = nullptr;
     SliceResult* out = nullptr;
     int state = 0;  // 0 free, 1 start pending, 2 active
@@ -652,7 +671,43 @@ struct LanczosGroup::Impl {
   std::vector<std::unique_ptr<Slot>> slots;
   DevBuf gt_part, hbuf, scal_d, vrand, parts;
   int kcap = 0, scap = 0;
-  BlockWork bw;
+  BlockWork bw32, bw64;  // the FP32 and the FP64 blocked filter passes
+  // filters the listed slots' columns in[j] -> out[j], one pass per precision
+  int filter(const std::vector<int>& ids, const std::vector<const double*>& in,
+             const std::vector<double*>& out) {
+    for (int pr = 0; pr < 2; ++pr) {
+      std::vector<const double*> ii;
+      std::vector<double*> oo;
+      std::vector<const polparams*> pp;
+      for (size_t j = 0; j < ids.size(); ++j)
+        if (slots[ids[j]]->f32 == (pr == 1)) {
+          ii.push_back(in[j]);
+          oo.push_back(out[j]);
+          pp.push_back(slots[ids[j]]->L.pol);
+        }
+      if (ii.empty()) continue;
+      if (int e = filter_columns((int)ii.size(), ii.data(), oo.data(), pp.data(), pr == 1,
+                                 pr == 1 ? bw32 : bw64))
+        return e;
+    }
+    return 0;
+  }
+  // (re)starts slot s: a fresh recurrence from its start vector
+  int start(int s, bool f32) {
+    Slot& sl = *slots[s];
+    const int n = ctx().n;
+    sl.L.dT.assign(sl.maxit, 0.0);
+    sl.L.eT.assign(sl.maxit, 0.0);
+    sl.L.beta = sl.L.wn = sl.L.tr0 = sl.L.old2 = sl.L.nrm = 0.0;
+    sl.L.nwn = sl.L.kdim = 0;
+    sl.L.last_nconv = -1;
+    sl.L.active = true;
+    sl.k = 0;
+    sl.f32 = f32;
+    sl.state = 1;
+    EVSL_HIP(sl.L.V.alloc((size_t)n * (sl.maxit + 1) * sizeof(double)));
+    return 0;
+  }
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   size_t part_stride() const {
     const int n = ctx().n;
@@ -725,14 +780,10 @@ int LanczosGroup::add(const double* intv, int maxit, double tol, const double* v
   std::memcpy(sl.L.intv, intv, 4 * sizeof(double));
   sl.L.pol = const_cast<polparams*>(pol);
   sl.maxit = std::min(c.n, maxit);
-  sl.k = 0;
   sl.tol = tol;
   sl.v0 = v0;
   sl.out = out;
-  EVSL_HIP(sl.L.V.alloc((size_t)c.n * (sl.maxit + 1) * sizeof(double)));
-  sl.L.dT.assign(sl.maxit, 0.0);
-  sl.L.eT.assign(sl.maxit, 0.0);
-  sl.state = 1;
+  if (int e = g.start(s, g.mixed && pol->deg >= c.mixed_min_deg)) return e;
   return g.ensure(sl.maxit);
 }
 
@@ -741,7 +792,6 @@ int LanczosGroup::step() {
   Context& c = ctx();
   const int n = c.n;
   FILE* fstats = g.fstats;
-  const bool mixed = g.mixed, f32 = mixed;
   if (!g.ev[0])
     for (auto& e : g.ev) EVSL_HIP(hipEventCreate(&e));
   auto elapsed = [](hipEvent_t a, hipEvent_t b) {
@@ -761,15 +811,12 @@ int LanczosGroup::step() {
     if (!st.empty()) {
       std::vector<const double*> in;
       std::vector<double*> out;
-      std::vector<const polparams*> pp;
       for (int s : st) {
         in.push_back(g.slots[s]->v0);
         out.push_back(g.slots[s]->L.vcol(0, n));
-        pp.push_back(g.slots[s]->L.pol);
       }
       EVSL_HIP(hipEventRecord(g.ev[0], c.stream));
-      if (int e = filter_columns((int)in.size(), in.data(), out.data(), pp.data(), f32, g.bw))
-        return e;
+      if (int e = g.filter(st, in, out)) return e;
       EVSL_HIP(hipEventRecord(g.ev[1], c.stream));
       c.astats.t_filter += elapsed(g.ev[0], g.ev[1]);
       for (int s : st) {
@@ -783,7 +830,6 @@ int LanczosGroup::step() {
         EVSL_HIP(evsl_dev::scal(n, 1.0 / std::sqrt(nv[4 * s]), g.slots[s]->L.vcol(0, n),
                                 c.stream));
         g.slots[s]->state = 2;
-        g.slots[s]->v0 = nullptr;
       }
     }
   }
@@ -796,15 +842,13 @@ int LanczosGroup::step() {
   {
     std::vector<const double*> in(b);
     std::vector<double*> out(b);
-    std::vector<const polparams*> pp(b);
     for (int j = 0; j < b; ++j) {
       Impl::Slot& sl = *g.slots[act[j]];
       in[j] = sl.L.vcol(sl.k, n);
       out[j] = sl.L.vcol(sl.k + 1, n);
-      pp[j] = sl.L.pol;
     }
     EVSL_HIP(hipEventRecord(g.ev[0], c.stream));
-    if (int e = filter_columns(b, in.data(), out.data(), pp.data(), f32, g.bw)) return e;
+    if (int e = g.filter(act, in, out)) return e;
     EVSL_HIP(hipEventRecord(g.ev[1], c.stream));
   }
   // ---- recurrence: vnew -= beta vold; alpha; vnew -= alpha v; ||vnew||^2 -----
@@ -871,7 +915,7 @@ int LanczosGroup::step() {
     if (L.beta * L.nwn < kOrthTol * L.wn) {  // lucky breakdown: new filtered random vector
       rand_double_device(n, g.vrand.as<double>());
       ChebWork cw;
-      if (int e = cheb_av(L.pol, g.vrand.as<double>(), vnew, cw, mixed ? EVSL_PREC_MIXED : 0))
+      if (int e = cheb_av(L.pol, g.vrand.as<double>(), vnew, cw, sl.f32 ? EVSL_PREC_MIXED : 0))
         return e;
       double d2[2];
       EVSL_HIP(evsl_dev::dot_partial(n, vnew, vnew, red_part(), c.stream));
@@ -942,8 +986,8 @@ int LanczosGroup::step() {
     // ~1e-6, so the trace is compared at that level and the count of Ritz
     // values above bar must also be unchanged.
     const double dtr = std::fabs(tr1 - L.tr0);
-    bool done = mixed ? (dtr < 1e-7 * std::max(1, nconv) && nconv == L.last_nconv)
-                      : (dtr < 2e-12 || std::fabs(tr1) + std::fabs(L.tr0) < 1e-10);
+    bool done = sl.f32 ? (dtr < 1e-7 * std::max(1, nconv) && nconv == L.last_nconv)
+                       : (dtr < 2e-12 || std::fabs(tr1) + std::fabs(L.tr0) < 1e-10);
     if (fstats)
       std::fprintf(fstats, "slice %d k %4d: nconv %4d  tr1 %21.15e dtr %.3e\n", s, k, nconv, tr1,
                    dtr);
@@ -954,9 +998,22 @@ int LanczosGroup::step() {
       const double tr = evsl_timer();
       if (fstats) std::fprintf(fstats, "slice %d done at k %d\n", s, k);
       SliceResult* o = sl.out;
+      bool stalled = false;
       int err = refine_extract(n, L.kdim, L.dT.data(), L.eT.data(), L.V.as<double>(), L.pol,
-                               L.intv, sl.tol, mixed, &o->nev, &o->lam, &o->W, &o->res, fstats);
+                               L.intv, sl.tol, sl.f32, &o->nev, &o->lam, &o->W, &o->res, fstats,
+                               &stalled);
       c.astats.t_ritz += evsl_timer() - tr;
+      if (!err && stalled) {
+        // the FP32-filtered basis cannot reach tol: the slice again in FP64
+        if (fstats) std::fprintf(fstats, "slice %d: refinement stalled, rerun in FP64\n", s);
+        std::free(o->lam);
+        std::free(o->res);
+        if (o->W) _evsl_Free_device(o->W);
+        *o = SliceResult();
+        ++c.astats.n_refine_extra;
+        if (int e = g.start(s, false)) return e;
+        continue;
+      }
       L.V.release();
       o->err = err;
       o->steps = L.kdim;

@@ -90,6 +90,7 @@ struct Context {
   int filter_prec = EVSL_PREC_FP64;
   int refine_passes = 1;
   double refine_guard = 0.6;
+  int mixed_min_deg = 32;  // mixed form: filters below this degree run in FP64
   // scratch for reductions
   DevBuf red_part, red_scalars;
   // pinned host scalars for device->host reads
@@ -191,7 +192,8 @@ struct SliceResult {
 // Slices of the current matrix (shared lmin, lmax) advancing through the
 // filtered Lanczos recurrence together, one step at a time; a slice added
 // between steps joins with the next one.  pol and v0 must stay valid until
-// the slice is done (v0 until its first step).
+// the slice is done (a mixed slice whose refinement stalls restarts from v0
+// in FP64).
 class LanczosGroup {
  public:
   LanczosGroup(bool mixed, FILE* fstats);

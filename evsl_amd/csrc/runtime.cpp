@@ -543,6 +543,11 @@ int evsl_amd_set_refine_guard(double guard) {
   ctx().refine_guard = guard;
   return 0;
 }
+int evsl_amd_set_mixed_min_degree(int deg) {
+  if (deg < 0) return EVSL_ERR_ARG;
+  ctx().mixed_min_deg = deg;
+  return 0;
+}
 int evsl_amd_get_stats(evslAmdStats* s) {
   if (!s) return EVSL_ERR_ARG;
   *s = ctx().astats;

@@ -94,3 +94,38 @@ def test_batch_rejects_mixed_bounds(ev):
     assert err == -104  # EVSL_ERR_ARG
     ev.free_pol(p1)
     ev.free_pol(p2)
+
+
+def test_mixed_low_degree_end_interval_matches_fp64(ev):
+    """An end interval of a random symmetric matrix gets a degree-4 filter,
+    too weak for the FP32-filtered basis plus refinement to reach tol.  With
+    the default minimum degree the slice runs in FP64 under the mixed form;
+    with the minimum lowered to 1 it runs mixed, its refinement stalls, and it
+    is redone from its start vector in FP64.  Either way the result equals
+    the FP64 run bit for bit."""
+    A = M.random_symmetric(39817, 8, 32, seed=487532661)
+    lo, hi = M.gershgorin(A)
+    assert ev.set_matrix(A) == 0
+    iv = np.array([-35.942844323629906, -10.02213023817233, lo, hi])
+    pol = ev.default_pol()
+    pol.damping, pol.thresh_int, pol.thresh_ext, pol.max_deg = 2, 0.8, 0.2, 3000
+    assert ev.find_pol(iv, pol) == 0
+    assert pol.deg < 32, pol.deg
+    v0 = np.random.default_rng(3).uniform(-1, 1, A.n)
+    ev.set_precision(PREC_FP64)
+    e64, (r64,) = ev.ChebLanNr_batch([iv], 300, 1e-8, v0, [pol])
+    assert e64 == 0 and len(r64[0]) > 0
+    ev.set_precision(PREC_MIXED)
+    try:
+        for mindeg in (32, 1):
+            assert ev.lib.evsl_amd_set_mixed_min_degree(mindeg) == 0
+            ev.reset_stats()
+            em, (rm,) = ev.ChebLanNr_batch([iv], 300, 1e-8, v0, [pol])
+            assert em == 0, ev.last_error()
+            assert np.array_equal(rm[0], r64[0]) and np.array_equal(rm[2], r64[2]), mindeg
+            if mindeg == 1:
+                assert ev.stats()["n_refine_extra"] >= 1   # the stalled slice was redone
+    finally:
+        ev.lib.evsl_amd_set_mixed_min_degree(32)
+        ev.set_precision(PREC_FP64)
+        ev.free_pol(pol)
