@@ -648,13 +648,16 @@ int refine_extract(int n, int kdim, const double* dT, const double* eT, const do
 // result) does not depend on which other slices share the group or when they
 // joined.
 //
-// In the mixed form a slice filters in FP32 only when its polynomial has a
-// minimum degree: a lower-degree filter carries few of the bytes (the
-// re-orthogonalisation dominates) and separates the refinement's guard set
-// too weakly (an end interval's degree-4 filter left every pair at ~2e-7
-// after three refinement passes); the minimum is set by
-// evsl_amd_set_mixed_min_degree (default 32).  A mixed slice whose refinement still
-// stalls above tol is run again from its start vector in FP64.
+// In the mixed form a slice filters in FP32 only when its polynomial degree
+// lies in a range (evsl_amd_set_mixed_degree_range, default [32, 3000]): a
+// lower-degree filter carries few of the bytes (the re-orthogonalisation
+// dominates) and separates the refinement's guard set too weakly (an end
+// interval's degree-4 filter left every pair at ~2e-7 after three refinement
+// passes); at a much higher degree the FP32 residual recurrence of the
+// refinement accumulates rounding like k^1.5 eps_fp32 relative to the
+// residual (~4% at degree 7250, where 2 of 16 slices stalled).  A mixed slice
+// whose refinement still stalls above tol is run again from its start vector
+// in FP64.
 
 struct LanczosGroup::Impl {
   struct Slot {
@@ -783,7 +786,8 @@ int LanczosGroup::add(const double* intv, int maxit, double tol, const double* v
   sl.tol = tol;
   sl.v0 = v0;
   sl.out = out;
-  if (int e = g.start(s, g.mixed && pol->deg >= c.mixed_min_deg)) return e;
+  if (int e = g.start(s, g.mixed && pol->deg >= c.mixed_min_deg && pol->deg <= c.mixed_max_deg))
+    return e;
   return g.ensure(sl.maxit);
 }
 

@@ -90,7 +90,8 @@ struct Context {
   int filter_prec = EVSL_PREC_FP64;
   int refine_passes = 1;
   double refine_guard = 0.6;
-  int mixed_min_deg = 32;  // mixed form: filters below this degree run in FP64
+  int mixed_min_deg = 32;    // mixed form: filters of degree outside
+  int mixed_max_deg = 3000;  // [mixed_min_deg, mixed_max_deg] run in FP64
   // scratch for reductions
   DevBuf red_part, red_scalars;
   // pinned host scalars for device->host reads

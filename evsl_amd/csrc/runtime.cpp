@@ -543,9 +543,10 @@ int evsl_amd_set_refine_guard(double guard) {
   ctx().refine_guard = guard;
   return 0;
 }
-int evsl_amd_set_mixed_min_degree(int deg) {
-  if (deg < 0) return EVSL_ERR_ARG;
-  ctx().mixed_min_deg = deg;
+int evsl_amd_set_mixed_degree_range(int min_deg, int max_deg) {
+  if (min_deg < 0 || max_deg < min_deg) return EVSL_ERR_ARG;
+  ctx().mixed_min_deg = min_deg;
+  ctx().mixed_max_deg = max_deg;
   return 0;
 }
 int evsl_amd_get_stats(evslAmdStats* s) {

@@ -46,7 +46,7 @@ EXPORTS = [
     "evsl_amd_server_create", "evsl_amd_server_destroy", "evsl_amd_server_add_matrix",
     "evsl_amd_server_submit_slice", "evsl_amd_server_submit_dos", "evsl_amd_server_step",
     "evsl_amd_server_status", "evsl_amd_server_slice_result", "evsl_amd_server_dos_result",
-    "evsl_amd_server_stats", "evsl_amd_set_mixed_min_degree",
+    "evsl_amd_server_stats", "evsl_amd_set_mixed_degree_range",
 ]
 
 
@@ -119,7 +119,7 @@ class EVSL:
             "evsl_amd_get_filter_precision": (I, []),
             "evsl_amd_set_refine_passes": (I, [I]),
             "evsl_amd_set_refine_guard": (I, [D]),
-            "evsl_amd_set_mixed_min_degree": (I, [I]),
+            "evsl_amd_set_mixed_degree_range": (I, [I, I]),
             "evsl_amd_get_stats": (I, [C.POINTER(AmdStats)]),
             "evsl_amd_reset_stats": (V, []),
             "evsl_amd_last_error": (C.c_char_p, []),

@@ -50,12 +50,14 @@ int evsl_amd_set_refine_passes(int passes);
  * guard * bar enter the FP64 Rayleigh-Ritz space (0 < guard <= 1,
  * default 0.6) */
 int evsl_amd_set_refine_guard(double guard);
-/* MIXED mode: slices whose filter has degree below deg run in FP64 (default
- * 32: a low-degree filter carries few of the bytes and separates the
- * refinement's guard set too weakly); a mixed slice whose refinement stalls
- * above tol after its extra passes is redone from its start vector in FP64
- * (counted in evslAmdStats.n_refine_extra). */
-int evsl_amd_set_mixed_min_degree(int deg);
+/* MIXED mode: only slices whose filter degree lies in [min_deg, max_deg]
+ * filter in FP32 (default [32, 3000]); the others run in FP64 -- a
+ * low-degree filter carries few of the bytes and separates the refinement's
+ * guard set too weakly, and at very high degrees the refinement's FP32
+ * residual recurrence accumulates too much rounding.  A mixed slice whose
+ * refinement stalls above tol after its extra passes is redone from its
+ * start vector in FP64 (counted in evslAmdStats.n_refine_extra). */
+int evsl_amd_set_mixed_degree_range(int min_deg, int max_deg);
 
 int evsl_amd_get_stats(evslAmdStats *s);
 void evsl_amd_reset_stats(void);

@@ -118,7 +118,7 @@ def test_mixed_low_degree_end_interval_matches_fp64(ev):
     ev.set_precision(PREC_MIXED)
     try:
         for mindeg in (32, 1):
-            assert ev.lib.evsl_amd_set_mixed_min_degree(mindeg) == 0
+            assert ev.lib.evsl_amd_set_mixed_degree_range(mindeg, 3000) == 0
             ev.reset_stats()
             em, (rm,) = ev.ChebLanNr_batch([iv], 300, 1e-8, v0, [pol])
             assert em == 0, ev.last_error()
@@ -126,6 +126,6 @@ def test_mixed_low_degree_end_interval_matches_fp64(ev):
             if mindeg == 1:
                 assert ev.stats()["n_refine_extra"] >= 1   # the stalled slice was redone
     finally:
-        ev.lib.evsl_amd_set_mixed_min_degree(32)
+        ev.lib.evsl_amd_set_mixed_degree_range(32, 3000)
         ev.set_precision(PREC_FP64)
         ev.free_pol(pol)
