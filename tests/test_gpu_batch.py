@@ -129,3 +129,41 @@ def test_mixed_low_degree_end_interval_matches_fp64(ev):
         ev.lib.evsl_amd_set_mixed_degree_range(32, 3000)
         ev.set_precision(PREC_FP64)
         ev.free_pol(pol)
+
+
+def test_degenerate_spectrum_mixed_pairs_are_true_eigenpairs(ev):
+    """Unperturbed 3-D Laplacian: eigenvalues of multiplicity up to 6.  A
+    single-start-vector Lanczos in FP64 finds one copy of each multiple
+    eigenvalue; the FP32 filter's rounding lets the mixed form find further
+    copies.  Every pair it returns must be a true one: each eigenvalue within
+    1e-10 of a closed-form eigenvalue, no value more often than its
+    multiplicity, orthonormal vectors, residuals below tol."""
+    nx = 22
+    A = M.laplacian(nx, nx, nx)
+    a, b = 0.9, 1.2
+    exact = M.laplacian_eigs(nx, nx, nx, a, b)
+    ivs, pols, mlan = setup_slices(ev, A, a, b, 0.0, 12.0, 2)
+    v0 = np.random.default_rng(5).uniform(-1, 1, A.n)
+    ev.set_precision(PREC_FP64)
+    e64, r64 = ev.ChebLanNr_batch(ivs, mlan, 1e-8, v0, pols)
+    ev.set_precision(PREC_MIXED)
+    emx, rmx = ev.ChebLanNr_batch(ivs, mlan, 1e-8, v0, pols)
+    ev.set_precision(PREC_FP64)
+    assert e64 == 0 and emx == 0, ev.last_error()
+    S = A.to_scipy()
+    uniq, mult = np.unique(np.round(exact, 10), return_counts=True)
+    for (l64, _, _), (lm, Ym, rm) in zip(r64, rmx):
+        print(f"  slice: FP64 {len(l64)} pairs, mixed {len(lm)}", flush=True)
+        idx = np.searchsorted(uniq, lm)
+        idx = np.clip(idx, 0, len(uniq) - 1)
+        near = np.where(np.abs(uniq[idx] - lm) < np.abs(uniq[np.maximum(idx - 1, 0)] - lm),
+                        idx, np.maximum(idx - 1, 0))
+        assert np.all(np.abs(uniq[near] - lm) < 1e-10 * np.maximum(1.0, np.abs(lm)))
+        used = np.bincount(near, minlength=len(uniq))
+        assert np.all(used <= mult)
+        assert np.all(rm < 1e-8)
+        assert np.abs(Ym.T @ Ym - np.eye(len(lm))).max() < 1e-8
+        R = S @ Ym - Ym * lm
+        assert np.all(np.linalg.norm(R, axis=0) < 1e-8)
+    for p in pols:
+        ev.free_pol(p)
