@@ -24,6 +24,8 @@ inline int width_slot(int B) { return B <= 8 ? B - 1 : (B == 16 ? 8 : (B == 32 ?
 inline int width_of_slot(int s) { return s < 8 ? s + 1 : (s == 8 ? 16 : 32); }
 inline int rows_cap_for(int B) { return kWG / B; }
 
+struct SpmmTuner;
+
 struct Csr {
   int n = 0, nnz = 0;
   const int* ia = nullptr;
@@ -36,6 +38,8 @@ struct Csr {
   // per block width: nblk+1 pairs (first row of the block, its row pointer)
   const int* rb[kNumWidths] = {};
   int nblk[kNumWidths] = {};
+  // the matrix's layout tuner (host object; null: the fixed layout policy)
+  SpmmTuner* tune = nullptr;
 };
 
 enum Mode { kPlain = 0, kCheb = 1, kKpm = 2, kRes = 3 };
@@ -60,6 +64,67 @@ hipError_t cheb_spmm(const Csr& A, int B, bool fp32, Mode mode, const void* xk,
                      const void* xkm1, void* xkp1, void* y, const void* v0,
                      const double* s, const double* mu0, double t, double cc,
                      bool first, int yflush, double* part, hipStream_t st);
+
+// Lane layouts of the fused filter kernel, as 10*V + R (V columns per lane,
+// R rows per lane group), that the layout tuner chooses among for the ChebAv
+// and residual steps of kTuneWidths columns; 0 ends a row.  Every layout
+// sums each row in the same order, so all give the same bits.
+constexpr int kTuneWidths[4] = {4, 8, 16, 32};
+constexpr int kTuneCand[2][4][6] = {{{11, 12, 14, 21, 22, 0},     // FP64, B = 4
+                                     {11, 12, 21, 22, 24, 0},     // 8
+                                     {11, 12, 21, 22, 24, 0},     // 16
+                                     {11, 12, 21, 22, 24, 0}},    // 32
+                                    {{11, 12, 14, 21, 22, 41},    // FP32, B = 4
+                                     {11, 21, 22, 24, 41, 42},    // 8
+                                     {11, 21, 22, 41, 42, 44},    // 16
+                                     {11, 21, 22, 41, 42, 44}}};  // 32
+inline int tune_width_index(int B) {
+  return B == 4 ? 0 : B == 8 ? 1 : B == 16 ? 2 : B == 32 ? 3 : -1;
+}
+// the tuner's candidates for (B, precision): count, *list (null if none)
+inline int tune_candidates(int B, bool fp32, const int** list) {
+  const int w = tune_width_index(B);
+  if (w < 0) {
+    if (list) *list = nullptr;
+    return 0;
+  }
+  const int* row = kTuneCand[fp32 ? 1 : 0][w];
+  int k = 0;
+  while (k < 6 && row[k]) ++k;
+  if (list) *list = row;
+  return k;
+}
+// layout (10*V + R) of the fixed policy for (B, precision, mode)
+int spmm_table_layout(int B, bool fp32, Mode mode);
+// one cheb_spmm launch with an explicit candidate layout (kCheb or kRes, B
+// in kTuneWidths); hipErrorInvalidValue for a layout not in the list
+hipError_t cheb_spmm_layout(int layout, const Csr& A, int B, bool fp32, Mode mode,
+                            const void* xk, const void* xkm1, void* xkp1, void* y,
+                            const void* v0, const double* s, const double* mu0, double t,
+                            double cc, bool first, int yflush, double* part, hipStream_t st);
+// The per-matrix layout tuner (spmm_tune.cpp).  While a (precision, mode,
+// width) key is undecided, each launch of it runs one candidate between two
+// events (the ChebAv/residual launches with yflush 0 or 3, the steady state
+// of a long series); once every candidate holds its samples and the events
+// have completed, the key keeps the candidate of least time per series step
+// (two yflush-0 launches and one yflush-3 launch).  Layouts give the same
+// bits, so tuning never changes a result.
+SpmmTuner* spmm_tuner_create();
+void spmm_tuner_destroy(SpmmTuner* t);
+// policy of a key: layout > 0 forces that candidate, 0 the fixed table,
+// -1 tunes (the default); resets what was learned.  -1 on a bad key/layout.
+int spmm_tuner_set(SpmmTuner* t, int B, bool fp32, Mode mode, int layout);
+// layout the key launches now: the forced or chosen one, or the table's
+// while undecided
+int spmm_tuner_layout(SpmmTuner* t, int B, bool fp32, Mode mode);
+// 1 when the key launches a fixed layout (forced, table, or tuned and
+// decided), 0 while it is still measuring
+int spmm_tuner_decided(SpmmTuner* t, int B, bool fp32, Mode mode);
+// cheb_spmm through the tuner (called by cheb_spmm for tunable keys)
+hipError_t cheb_spmm_tuned(const Csr& A, int B, bool fp32, Mode mode, const void* xk,
+                           const void* xkm1, void* xkp1, void* y, const void* v0,
+                           const double* s, const double* mu0, double t, double cc,
+                           bool first, int yflush, double* part, hipStream_t st);
 
 // y terms flushed by launch k of a degree-m series (k = 1..m): the first
 // launch sets y, then every third launch adds three terms, and the last adds

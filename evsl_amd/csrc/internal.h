@@ -59,6 +59,7 @@ struct DevMatrix {
   bool full_diag = false;          // every row stores its diagonal entry
   int* rb[evsl_dev::kNumWidths] = {};
   int nblk[evsl_dev::kNumWidths] = {};
+  evsl_dev::SpmmTuner* tune = nullptr;  // the filter kernel's layout tuner
   bool owns_csr = false;  // ia/ja/a64 allocated here (SetAMatrix)
   // bytes per nonzero of the column index the filter kernels read
   int idx_bytes() const { return jd ? 2 : 4; }
@@ -92,6 +93,7 @@ struct Context {
   double refine_guard = 0.6;
   int mixed_min_deg = 32;    // mixed form: filters of degree outside
   int mixed_max_deg = 3000;  // [mixed_min_deg, mixed_max_deg] run in FP64
+  bool spmm_tuning = true;   // filter launches go through the matrix's layout tuner
   // scratch for reductions
   DevBuf red_part, red_scalars;
   // pinned host scalars for device->host reads

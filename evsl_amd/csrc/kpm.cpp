@@ -274,6 +274,16 @@ int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, doub
       mode == 2 ? evsl_dev::kKpm : (mode == 3 ? evsl_dev::kRes : evsl_dev::kCheb);
   // steady state of a long series: y is flushed (3 terms) every third launch
   nsteps = (nsteps + 2) / 3 * 3;
+  // the layout tuner settles first (its measuring launches run candidate
+  // layouts), so the timed launches run the layout the solvers will use
+  if (md != evsl_dev::kKpm && op.A.tune) {
+    for (int it = 0; it < 4096 && !evsl_dev::spmm_tuner_decided(op.A.tune, B, f32, md); ++it) {
+      EVSL_HIP(evsl_dev::cheb_spmm(op.A, B, f32, md, buf[it % 3], buf[(it + 1) % 3],
+                                   buf[(it + 2) % 3], buf[3], buf[4], scol, mu0col, op.t2, op.cc,
+                                   false, it % 3 == 2 ? 3 : 0, red_part(), c.stream));
+      if (it % 64 == 63) EVSL_HIP(hipStreamSynchronize(c.stream));
+    }
+  }
   hipEvent_t e0, e1;
   EVSL_HIP(hipEventCreate(&e0));
   EVSL_HIP(hipEventCreate(&e1));
@@ -311,10 +321,42 @@ int evsl_amd_spmm_index_bytes(void) {
   return c.op == 1 ? c.A.idx_bytes() : 0;
 }
 
-int evsl_amd_spmm_vec(int B, int prec, int mode) {
+int evsl_amd_spmm_layout(int B, int prec, int mode, int* decided) {
+  const Context& c = ctx();
   const evsl_dev::Mode md =
       mode == 2 ? evsl_dev::kKpm : (mode == 3 ? evsl_dev::kRes : evsl_dev::kCheb);
-  return evsl_dev::cheb_spmm_vec(B, prec == EVSL_PREC_MIXED && mode != 2, md);
+  const bool f32 = prec == EVSL_PREC_MIXED && mode != 2;
+  if (decided) *decided = 1;
+  if (mode < 1 || mode > 3) return -1;
+  if (c.op != 1 || !c.A.tune || !c.spmm_tuning) return evsl_dev::spmm_table_layout(B, f32, md);
+  if (decided) *decided = evsl_dev::spmm_tuner_decided(c.A.tune, B, f32, md);
+  return evsl_dev::spmm_tuner_layout(c.A.tune, B, f32, md);
+}
+
+int evsl_amd_set_spmm_layout(int B, int prec, int mode, int layout) {
+  const Context& c = ctx();
+  if (c.op != 1 || !c.A.tune) {
+    set_error_msg("evsl_amd_set_spmm_layout: no device matrix (SetAMatrix first)");
+    return EVSL_ERR_NOMAT;
+  }
+  if (mode != 1 && mode != 3) return EVSL_ERR_ARG;
+  const evsl_dev::Mode md = mode == 3 ? evsl_dev::kRes : evsl_dev::kCheb;
+  if (evsl_dev::spmm_tuner_set(c.A.tune, B, prec == EVSL_PREC_MIXED, md, layout) != 0) {
+    set_error_msg("evsl_amd_set_spmm_layout: B must be 4, 8, 16 or 32 and the layout one of "
+                  "its candidates, 0 or -1");
+    return EVSL_ERR_ARG;
+  }
+  return 0;
+}
+
+int evsl_amd_set_spmm_tuning(int on) {
+  ctx().spmm_tuning = on != 0;
+  return 0;
+}
+
+int evsl_amd_spmm_vec(int B, int prec, int mode) {
+  const int L = evsl_amd_spmm_layout(B, prec, mode, nullptr);
+  return L > 0 ? L / 10 : -1;
 }
 
 int evsl_amd_spmm_streaming(int B, int prec, int mode) {

@@ -60,6 +60,7 @@ evsl_dev::Csr DevMatrix::view() const {
     c.rb[w] = rb[w];
     c.nblk[w] = nblk[w];
   }
+  c.tune = ctx().spmm_tuning ? tune : nullptr;
   return c;
 }
 
@@ -74,6 +75,7 @@ void DevMatrix::release() {
   if (jd) (void)hipFree(jd);
   if (a32) (void)hipFree(a32);
   if (a32s) (void)hipFree(a32s);
+  evsl_dev::spmm_tuner_destroy(tune);
   *this = DevMatrix();
 }
 
@@ -186,6 +188,11 @@ int build_device_matrix(int n, int nnz, int* ia_d, int* ja_d, double* a_d, const
     full = has;
   }
   M.full_diag = full;
+  M.tune = evsl_dev::spmm_tuner_create();
+  if (!M.tune) {
+    set_error_msg("out of host memory for the layout tuner");
+    return EVSL_ERR_DEVICE;
+  }
   // 16-bit column offsets when every |ja - row| fits (banded and stencil
   // matrices in a local ordering): 2 bytes less per nonzero and filter step
   {

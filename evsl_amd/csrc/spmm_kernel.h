@@ -1,0 +1,432 @@
+// spmm_kernel.h -- the fused Chebyshev-step CSR SpMM kernel (device code),
+// its explicit-layout launcher and the fixed layout policy.  Included by the
+// HIP translation units only: kernels.hip (fixed-policy launches, grouped
+// KPM) and spmm_layouts_f32.hip / spmm_layouts_f64.hip (every candidate
+// layout of the per-matrix layout tuner).  Everything sits in an unnamed
+// namespace, so each unit holds its own instances.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "dev.h"
+
+namespace evsl_dev {
+namespace {
+
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+  const int xcd = bid % kXcds, pos = bid / kXcds;
+  const int per = nblk / kXcds, rem = nblk % kXcds;
+  return xcd * per + min(xcd, rem) + pos;
+}
+
+// ---------------------------------------------------------------------
+// fused Chebyshev step SpMM
+// ---------------------------------------------------------------------
+// V contiguous values of one row of an interleaved block (one vector load)
+template <typename VT, int V>
+struct alignas(sizeof(VT) * V) VecT {
+  VT v[V];
+};
+
+// Lane layout: L = B/V lanes share a row, each owning V adjacent columns
+// (c0 = sub*V), so a block of 256 lanes covers NG = 256/L rows and every x
+// access is one V-wide vector load.  LDS = true stages the block's ja/a
+// through LDS with coalesced loads; LDS = false reads them from global memory
+// (the L lanes of a row load the same address).  The per-column summation
+// order is the same for every (V, LDS), so results are bitwise independent
+// of the variant (the KPM block partials aside, whose tree depends on NG).
+// NT marks the streams a launch touches last -- xkm1, the flushed y and the
+// xkp1 store -- non-temporal, so that they do not push the gathered x rows
+// out of L2; it pays when the launch's working set exceeds the Infinity
+// Cache (chosen per launch, streaming_launch below) and leaves the
+// arithmetic unchanged.
+template <typename T, int V>
+struct ExtVec {
+  typedef T type __attribute__((ext_vector_type(V)));
+};
+template <typename T>
+struct ExtVec<T, 1> {
+  typedef T type;
+};
+// The work of one row block `blk`: shared by the one-matrix kernel
+// (k_cheb_spmm) and the grouped KPM kernel (k_kpm_group), so that a block's
+// arithmetic is the same in both.
+template <typename VT, int B, int V, int MODE, bool LDS, bool I16, bool NT>
+__device__ __forceinline__ void cheb_spmm_block(
+    const int blk, const int* __restrict__ ia, const int* __restrict__ ja,
+    const short* __restrict__ jd, const VT* __restrict__ av, const int2* __restrict__ rbp,
+    const VT* __restrict__ xk, const VT* __restrict__ xkm1, VT* __restrict__ xkp1,
+    VT* __restrict__ y, const VT* __restrict__ v0, const double* __restrict__ scol,
+    const double* __restrict__ mu0col, double t_, double cc_, int first, int yflush,
+    double* __restrict__ part) {
+  static_assert(B % V == 0, "V must divide B");
+  constexpr int L = B / V;
+  constexpr int NG = kWG / L;  // lanes >= NG*L idle when L does not divide 256
+  constexpr int U = (V * sizeof(VT) >= 32) ? 4 : 8;  // gathers in flight per lane
+  using Vec = VecT<VT, V>;
+  // staging area for ja/a, reused for the group reductions once the row
+  // walk is over
+  // column indices: int32, or (I16) 16-bit offsets from the row, ja = row + jd
+  using IT = typename std::conditional<I16, short, int>::type;
+  const IT* __restrict__ jx;
+  if constexpr (I16)
+    jx = jd;
+  else
+    jx = ja;
+  constexpr size_t kStage = LDS ? (size_t)kNnzCap * (sizeof(IT) + sizeof(VT)) : 0;
+  constexpr size_t kRed = (size_t)kWG * V * sizeof(double);
+  __shared__ __align__(16) unsigned char smem[kStage > kRed ? kStage : kRed];
+  IT* s_ja = reinterpret_cast<IT*>(smem);
+  VT* s_a = reinterpret_cast<VT*>(smem + (size_t)kNnzCap * sizeof(IT));
+  double* s_red = reinterpret_cast<double*>(smem);
+
+  // block table entries carry (first row, its row pointer): one load for both ends
+  const int2 e0 = rbp[blk], e1 = rbp[blk + 1];
+  const int r0 = e0.x, r1 = e1.x, p0 = e0.y, p1 = e1.y;
+  const int nz = p1 - p0;
+  const int tid = threadIdx.x;
+  const int grp = tid / L, sub = tid % L, c0 = sub * V;
+  const bool act = tid < NG * L;
+  const VT t = (VT)t_, cc = (VT)cc_;
+  // kRes: v0 holds the residual block R and scol the per-column factors
+  // 2*T_k(theta_c); mu0col[0..1] = (mu_k, mu_{k-1}) shared by all columns.
+  // y accumulates the last yflush terms T_{k-1}, T_k, T_{k+1} of the series
+  // (0: y untouched); s, s1, s2 are their coefficients.  kCheb keeps the
+  // per-degree coefficient rows B apart (scol - B, scol - 2B), kRes the
+  // (mu_k, mu_1) pairs 2 apart (mu0col - 2, mu0col - 4).
+  VT s[V], s1[V], s2[V], mu0[V], g[V];
+#pragma unroll
+  for (int j = 0; j < V; ++j) {
+    s[j] = s1[j] = s2[j] = 0;
+    mu0[j] = 0;
+    g[j] = 0;
+    if (MODE == kCheb) {
+      s[j] = (VT)scol[c0 + j];
+      if (yflush >= 2) s1[j] = (VT)scol[c0 + j - B];
+      if (yflush >= 3) s2[j] = (VT)scol[c0 + j - 2 * B];
+      if (first) mu0[j] = (VT)mu0col[c0 + j];
+    } else if (MODE == kRes) {
+      g[j] = (VT)scol[c0 + j];
+      s[j] = (VT)mu0col[0];
+      if (yflush >= 2) s1[j] = (VT)mu0col[-2];
+      if (yflush >= 3) s2[j] = (VT)mu0col[-4];
+      if (first) mu0[j] = (VT)mu0col[1];
+    }
+  }
+  const bool ymode = MODE == kCheb || MODE == kRes;
+  const bool need_xr = first || MODE == kKpm || cc_ != 0.0 || (ymode && yflush >= 2);
+  double dacc[V];
+#pragma unroll
+  for (int j = 0; j < V; ++j) dacc[j] = 0.0;
+
+  // epilogue operands of a row, loaded ahead of the gathers
+  struct Ops {
+    Vec xr, km1, yo, rr, v;
+  };
+  using EV = typename ExtVec<VT, V>::type;
+  auto ld = [](const VT* p, size_t i) { return *reinterpret_cast<const Vec*>(p + i); };
+  // the same load / store, non-temporal when NT
+  auto ld_last = [](const VT* p, size_t i) {
+    if constexpr (NT) {
+      const EV e = __builtin_nontemporal_load(reinterpret_cast<const EV*>(p + i));
+      Vec r;
+      __builtin_memcpy(&r, &e, sizeof(Vec));
+      return r;
+    } else {
+      return *reinterpret_cast<const Vec*>(p + i);
+    }
+  };
+  auto st_last = [](VT* p, size_t i, const Vec& v) {
+    if constexpr (NT) {
+      EV e;
+      __builtin_memcpy(&e, &v, sizeof(Vec));
+      __builtin_nontemporal_store(e, reinterpret_cast<EV*>(p + i));
+    } else {
+      *reinterpret_cast<Vec*>(p + i) = v;
+    }
+  };
+  auto prefetch = [&](int r) {
+    Ops o{};
+    const size_t i = (size_t)r * B + c0;
+    if (MODE == kPlain) return o;
+    if (need_xr) o.xr = ld(xk, i);
+    if (xkm1) o.km1 = ld_last(xkm1, i);
+    if (ymode && yflush > 0 && !first) o.yo = ld_last(y, i);
+    if (MODE == kRes) o.rr = ld(v0, i);
+    if (MODE == kKpm) o.v = ld(v0, i);
+    return o;
+  };
+  auto epilogue = [&](int r, const VT* acc, const Ops& o) {
+    const size_t i = (size_t)r * B + c0;
+    Vec w;
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+      if (MODE == kPlain) {
+        w.v[j] = acc[j];
+        continue;
+      }
+      VT wj = t * (acc[j] - cc * o.xr.v[j]) - o.km1.v[j];
+      if (MODE == kRes) wj += g[j] * o.rr.v[j];
+      w.v[j] = wj;
+    }
+    st_last(xkp1, i, w);
+    if (ymode) {
+      if (yflush > 0) {
+        // the same additions, in the same order, as one update per degree
+        Vec yn;
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+          VT yv = first ? mu0[j] * o.xr.v[j] : o.yo.v[j];
+          if (yflush >= 3) yv = yv + s2[j] * o.km1.v[j];
+          if (yflush >= 2) yv = yv + s1[j] * o.xr.v[j];
+          yn.v[j] = yv + s[j] * w.v[j];
+        }
+        st_last(y, i, yn);
+      }
+    } else if (MODE == kKpm) {
+#pragma unroll
+      for (int j = 0; j < V; ++j) dacc[j] += (double)w.v[j] * (double)o.v.v[j];
+    }
+  };
+
+  if (nz <= kNnzCap) {
+    // row pointers and epilogue operands of this lane's first row go out
+    // with the staging loads
+    const int rf = r0 + grp;
+    int qf0 = 0, qf1 = 0;
+    Ops of{};
+    if (act && rf < r1) {
+      qf0 = ia[rf] - p0;
+      qf1 = ia[rf + 1] - p0;
+      of = prefetch(rf);
+    }
+    const IT* jsrc = jx + p0;
+    const VT* asrc = av + p0;
+    if (LDS) {
+      for (int p = tid; p < nz; p += kWG) {
+        s_ja[p] = jx[p0 + p];
+        s_a[p] = av[p0 + p];
+      }
+      __syncthreads();
+      jsrc = s_ja;
+      asrc = s_a;
+    }
+    for (int r = rf; act && r < r1; r += NG) {
+      int q0 = qf0, q1 = qf1;
+      Ops o = of;
+      if (r != rf) {
+        q0 = ia[r] - p0;
+        q1 = ia[r + 1] - p0;
+        o = prefetch(r);
+      }
+      VT acc[V];
+#pragma unroll
+      for (int j = 0; j < V; ++j) acc[j] = 0;
+      // U independent gathers in flight per lane; zero padding leaves the
+      // sequential summation order (and its rounding) unchanged
+      for (int q = q0; q < q1; q += U) {
+        int jj[U];
+        VT aa[U];
+        Vec xx[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const bool ok = q + u < q1;
+          jj[u] = ok ? (I16 ? r : 0) + (int)jsrc[q + u] : 0;
+          aa[u] = ok ? asrc[q + u] : (VT)0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (q + u < q1) {
+            xx[u] = ld(xk, (size_t)jj[u] * B + c0);
+          } else {
+#pragma unroll
+            for (int j = 0; j < V; ++j) xx[u].v[j] = (VT)0;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int j = 0; j < V; ++j) acc[j] += aa[u] * xx[u].v[j];
+      }
+      epilogue(r, acc, o);
+    }
+    if (MODE == kKpm && LDS) __syncthreads();  // s_red aliases the staging area
+  } else {
+    // one long row: all groups split its nonzeros, combine through LDS
+    VT acc[V];
+#pragma unroll
+    for (int j = 0; j < V; ++j) acc[j] = 0;
+    for (int p = p0 + grp; act && p < p1; p += NG) {
+      const VT a = av[p];
+      const Vec x = ld(xk, (size_t)((I16 ? r0 : 0) + (int)jx[p]) * B + c0);
+#pragma unroll
+      for (int j = 0; j < V; ++j) acc[j] += a * x.v[j];
+    }
+#pragma unroll
+    for (int j = 0; j < V; ++j) s_red[tid * V + j] = (double)acc[j];
+    __syncthreads();
+    if (act && grp == 0) {
+      VT tot[V];
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        double sj = 0.0;
+#pragma unroll 1
+        for (int gg = 0; gg < NG; ++gg) sj += s_red[(gg * L + sub) * V + j];
+        tot[j] = (VT)sj;
+      }
+      epilogue(r0, tot, prefetch(r0));
+    }
+    __syncthreads();
+  }
+
+  if (MODE == kKpm) {  // L is a power of two here (NG*L == kWG)
+    static_assert(MODE != kKpm || (kWG % L) == 0, "KPM blocks need B/V | 256");
+#pragma unroll
+    for (int j = 0; j < V; ++j) s_red[tid * V + j] = dacc[j];
+    __syncthreads();
+    for (int st = NG / 2; st > 0; st >>= 1) {
+      if (grp < st) {
+#pragma unroll
+        for (int j = 0; j < V; ++j) s_red[tid * V + j] += s_red[(tid + st * L) * V + j];
+      }
+      __syncthreads();
+    }
+    if (grp == 0) {
+#pragma unroll
+      for (int j = 0; j < V; ++j) part[(size_t)blk * B + c0 + j] = s_red[tid * V + j];
+    }
+  }
+}
+
+template <typename VT, int B, int V, int MODE, bool LDS, bool I16, bool NT = false>
+__global__ __launch_bounds__(kWG) void k_cheb_spmm(
+    const int* __restrict__ ia, const int* __restrict__ ja, const short* __restrict__ jd,
+    const VT* __restrict__ av,
+    const int2* __restrict__ rbp, int nblk, const VT* __restrict__ xk,
+    const VT* __restrict__ xkm1, VT* __restrict__ xkp1, VT* __restrict__ y,
+    const VT* __restrict__ v0, const double* __restrict__ scol,
+    const double* __restrict__ mu0col, double t_, double cc_, int first, int yflush,
+    double* __restrict__ part) {
+  cheb_spmm_block<VT, B, V, MODE, LDS, I16, NT>(xcd_remap(blockIdx.x, nblk), ia, ja, jd, av, rbp,
+                                                xk, xkm1, xkp1, y, v0, scol, mu0col, t_, cc_,
+                                                first, yflush, part);
+}
+
+// KPM steps of several matrices in one launch: global row block gb (after
+// the XCD remap) belongs to matrix owner[gb], whose entry names its CSR, its
+// row-block table, its vector blocks and its recurrence factors; step k
+// reads T_k and T_{k-1} from the matrix's ring and writes T_{k+1} and the
+// block's moment partials, as the one-matrix kKpm launch does.
+// Non-temporal streams (the NT instances) exist for the ChebAv and residual
+// steps of 8 or more columns.  They are chosen when one launch's working set
+// -- the matrix, and the vector blocks xk, xkm1, xkp1, y (and R) -- exceeds
+// 320 MB: beyond the 256 MB Infinity Cache the block no longer stays
+// resident from launch to launch, and keeping the last-use streams out of L2
+// saves the x rows re-read across planes.  Measured on the 100^3 Laplacian
+// (µs per launch, plain -> non-temporal): FP32 32-column ChebAv 137.5 -> 124.8
+// and residual step 149.0 -> 141.4, FP32 16-column residual 81.9 -> 71.4, FP64
+// 8/16/32-column ChebAv 71.5 -> 67.8, 131.5 -> 118.3, 242.2 -> 232.5, FP64
+// 8-column residual 85.0 -> 72.7; below the threshold the hints cost 3-30%
+// (FP32 8- and 16-column ChebAv at 174 / 302 MB, FP64 4-column at 197 MB).
+template <int MODE, int B>
+constexpr bool kStreamingCapable = (MODE == kCheb || MODE == kRes) && B >= 8;
+
+inline bool streaming_launch(const Csr& A, int B, int vsize, int mode) {
+  if (!((mode == kCheb || mode == kRes) && B >= 8)) return false;
+  const double streams = mode == kRes ? 5.0 : 4.0;
+  const double ws = streams * (double)A.n * B * vsize +
+                    (double)A.nnz * (vsize + (A.jd ? 2 : 4)) + 4.0 * (A.n + 1);
+  return ws > 320e6;
+}
+
+// One fused step with an explicit lane layout (V columns per lane, LDS
+// staging on/off, R rows per lane group); the row-block table is the one
+// built for B/V/R lanes per row.
+template <typename VT, int B, int V, int MODE, bool LDS, int R = 1>
+hipError_t launch_variant(const Csr& A, const void* xk, const void* xkm1, void* xkp1, void* y,
+                          const void* v0, const double* s, const double* mu0, double t,
+                          double cc, bool first, int yflush, double* part, hipStream_t st) {
+  const VT* av;
+  if constexpr (sizeof(VT) == 4)
+    av = A.a32;
+  else
+    av = A.a64;
+  if (!av) return hipErrorInvalidValue;
+  static_assert((B / V) % R == 0, "R must divide the lanes per row");
+  const int slot = width_slot(B / V / R);
+  if (slot < 0 || !A.rb[slot]) return hipErrorInvalidValue;
+  const int nblk = A.nblk[slot];
+  const int2* rb = reinterpret_cast<const int2*>(A.rb[slot]);
+  const int f = first ? 1 : 0;
+#define EVSL_SPMM_LAUNCH(I16_, NT_)                                                             \
+  hipLaunchKernelGGL((k_cheb_spmm<VT, B, V, MODE, LDS, I16_, NT_>), dim3(nblk), dim3(kWG), 0, \
+                     st, A.ia, A.ja, A.jd, av, rb, nblk, (const VT*)xk, (const VT*)xkm1,      \
+                     (VT*)xkp1, (VT*)y, (const VT*)v0, s, mu0, t, cc, f, yflush, part)
+  if constexpr (kStreamingCapable<MODE, B>) {
+    if (streaming_launch(A, B, (int)sizeof(VT), MODE)) {
+      if (A.jd)
+        EVSL_SPMM_LAUNCH(true, true);
+      else
+        EVSL_SPMM_LAUNCH(false, true);
+      return hipGetLastError();
+    }
+  }
+  if (A.jd)
+    EVSL_SPMM_LAUNCH(true, false);
+  else
+    EVSL_SPMM_LAUNCH(false, false);
+#undef EVSL_SPMM_LAUNCH
+  return hipGetLastError();
+}
+
+// Layout of the fused filter kernel for each (value size, block width,
+// mode), as 10*V + R: V columns per lane and R rows per lane group.  With
+// V > 1 a lane takes V adjacent columns, so a block of 256 lanes covers more
+// than 256/B rows; with R > 1 a block covers R times the rows its lane groups
+// can hold at once and each group walks R of them, amortising the block's
+// table loads, staging and barrier (the row-block table is then the one built
+// for (B/V)/R lanes per row).  Chosen by a sweep of every valid (V, R) on the
+// 7-point 100^3 Laplacian on gfx950 (LDS staging won for every width); odd
+// widths only admit V = R = 1.  Time saved by the chosen R over the best
+// R = 1 layout: FP32 ChebAv 7% at 4 columns, 9% at 8, 15% at 16, 19% at 32;
+// FP64 ChebAv 2% at 4, 7% at 8, 11% at 16 and 32; the FP32 residual step
+// 7-8% at 2, 4, 16 and 32.  The FP32 KPM and FP64 residual rows (neither is
+// on a solver's hot path) keep the layouts of an earlier V-only sweep.
+//                                    B =  2   4   6   8  16  32
+constexpr int kSpmmLayout[2][3][6] = {{{21, 22, 21, 22, 44, 44},    // FP32 ChebAv
+                                       {21, 21, 21, 21, 21, 41},    // FP32 KPM
+                                       {12, 14, 21, 22, 44, 44}},   // FP32 residual
+                                      {{21, 12, 21, 22, 24, 24},    // FP64 ChebAv
+                                       {11, 12, 21, 41, 41, 41},    // FP64 KPM
+                                       {21, 11, 21, 22, 21, 22}}};  // FP64 residual
+
+constexpr int spmm_layout(int vsize, int B, int mode) {
+  const int w = B == 2 ? 0 : B == 4 ? 1 : B == 6 ? 2 : B == 8 ? 3 : B == 16 ? 4 : B == 32 ? 5 : -1;
+  return w < 0 ? 11 : kSpmmLayout[vsize == 8][mode - kCheb][w];
+}
+constexpr int spmm_vec(int vsize, int B, int mode) { return spmm_layout(vsize, B, mode) / 10; }
+constexpr int spmm_rows(int vsize, int B, int mode) { return spmm_layout(vsize, B, mode) % 10; }
+
+template <typename VT, int B, int MODE>
+struct SpmmPolicy {
+  static constexpr int V = spmm_vec((int)sizeof(VT), B, MODE == kPlain ? kCheb : MODE);
+  static constexpr int R = spmm_rows((int)sizeof(VT), B, MODE == kPlain ? kCheb : MODE);
+  static constexpr bool LDS = true;
+};
+
+}  // namespace
+
+// explicit-layout launches (spmm_layouts.hip, one object per precision)
+hipError_t cheb_spmm_layout_f32(int layout, const Csr& A, int B, Mode mode, const void* xk,
+                                const void* xkm1, void* xkp1, void* y, const void* v0,
+                                const double* s, const double* mu0, double t, double cc,
+                                bool first, int yflush, double* part, hipStream_t st);
+hipError_t cheb_spmm_layout_f64(int layout, const Csr& A, int B, Mode mode, const void* xk,
+                                const void* xkm1, void* xkp1, void* y, const void* v0,
+                                const double* s, const double* mu0, double t, double cc,
+                                bool first, int yflush, double* part, hipStream_t st);
+
+}  // namespace evsl_dev

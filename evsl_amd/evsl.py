@@ -47,6 +47,7 @@ EXPORTS = [
     "evsl_amd_server_submit_slice", "evsl_amd_server_submit_dos", "evsl_amd_server_step",
     "evsl_amd_server_status", "evsl_amd_server_slice_result", "evsl_amd_server_dos_result",
     "evsl_amd_server_stats", "evsl_amd_set_mixed_degree_range",
+    "evsl_amd_spmm_layout", "evsl_amd_set_spmm_layout", "evsl_amd_set_spmm_tuning",
 ]
 
 
@@ -127,6 +128,9 @@ class EVSL:
             "evsl_amd_bench_filter": (I, [I, I, I, I, D, D, dp, dp]),
             "evsl_amd_bench_copy": (I, [C.c_size_t, I, dp]),
             "evsl_amd_spmm_vec": (I, [I, I, I]),
+            "evsl_amd_spmm_layout": (I, [I, I, I, C.POINTER(I)]),
+            "evsl_amd_set_spmm_layout": (I, [I, I, I, I]),
+            "evsl_amd_set_spmm_tuning": (I, [I]),
             "evsl_amd_spmm_index_bytes": (I, []),
             "evsl_amd_spmm_streaming": (I, [I, I, I]),
             "kpmdos_batch": (I, [I, C.POINTER(C.POINTER(CsrMat)), dp, I, I, I, I,
@@ -410,6 +414,20 @@ class EVSL:
 
     def spmm_vec(self, B, prec, mode):
         return self.lib.evsl_amd_spmm_vec(B, prec, mode)
+
+    def spmm_layout(self, B, prec, mode):
+        """(layout 10*V + R the key launches now on the current matrix,
+        decided)."""
+        d = C.c_int()
+        lay = self.lib.evsl_amd_spmm_layout(B, prec, mode, C.byref(d))
+        return lay, bool(d.value)
+
+    def set_spmm_layout(self, B, prec, mode, layout):
+        """layout > 0 forces a candidate, 0 the fixed policy, -1 tunes."""
+        return self.lib.evsl_amd_set_spmm_layout(B, prec, mode, layout)
+
+    def set_spmm_tuning(self, on):
+        return self.lib.evsl_amd_set_spmm_tuning(1 if on else 0)
 
     def spmm_kernel_name(self, B, prec, mode):
         """Template name of the fused filter kernel for (B, prec, mode) on

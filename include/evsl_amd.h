@@ -160,8 +160,33 @@ int evsl_amd_read_coo_mm(const char *path, int idxin, int idxout, cooMat *A);
 int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, double dd,
                           double *ms_per_step, double *bytes_per_step);
 
+/* Lane layout of the fused filter kernel (no counterpart in the reference;
+ * its CSR matvec has no layout).  A layout is 10*V + R: V adjacent columns
+ * per lane and R rows per lane group.  The ChebAv (mode 1) and residual
+ * (mode 3) steps of 4, 8, 16 and 32 columns choose theirs per matrix: each
+ * device matrix carries a tuner that times the candidate layouts on the
+ * solver's own first launches of a (B, prec, mode) key, between HIP events it
+ * polls without waiting, and then keeps the fastest.  Every layout sums each
+ * row in the same order, so tuning never changes a result.
+ *   evsl_amd_spmm_layout: the layout the key launches now on the current
+ *     matrix (the fixed policy's while the tuner is still measuring, and for
+ *     KPM steps and other widths); *decided (may be NULL) 1 once fixed;
+ *     -1 for a bad mode.
+ *   evsl_amd_set_spmm_layout: the key's policy on the current matrix --
+ *     layout > 0 forces that candidate, 0 the fixed policy, -1 tunes (the
+ *     default); resets what the key has measured.  EVSL_ERR_ARG for another
+ *     width or mode or a layout outside the candidates, EVSL_ERR_NOMAT
+ *     without a device matrix.
+ *   evsl_amd_set_spmm_tuning: 0 runs every matrix on the fixed policy, 1
+ *     (the default) tunes. */
+int evsl_amd_spmm_layout(int B, int prec, int mode, int *decided);
+int evsl_amd_set_spmm_layout(int B, int prec, int mode, int layout);
+int evsl_amd_set_spmm_tuning(int on);
+
 /* Columns per lane of the fused filter kernel evsl_amd_bench_filter times
- * for (B, prec, mode): its template instance is
+ * for (B, prec, mode) on the current matrix (the tuned layout's V, after the
+ * tuner has settled; bench_filter lets it settle before timing): its
+ * template instance is
  * k_cheb_spmm<float|double, B, V, mode, true, I16, NT> (I16 from
  * evsl_amd_spmm_index_bytes, NT from evsl_amd_spmm_streaming); -1 for an
  * unsupported B. */
