@@ -215,6 +215,8 @@ def kpm_batch_bench(args):
     sum_n = sum(A.n for A in mats)
     ev = EVSL()
     assert ev.start() == 0
+    if args.no_tune:
+        ev.set_spmm_tuning(False)
     ev.kpmdos_batch(mats[:2], ivs[:2], 4, 1, 2, VEC_RADEMACHER, seeds[:2])  # priming
     res = {}
     for name, prec in (("mixed", PREC_MIXED), ("fp64", PREC_FP64)):
@@ -244,7 +246,7 @@ def kpm_batch_bench(args):
                 for A in mats) * mdeg
         gbs[name] = b / res[name][0] / 1e9
     d = np.abs(res["mixed"][1] - res["fp64"][1])
-    print(json.dumps({
+    rec = {
         "workload": f"batched KPM: {len(mats)} DOS requests over {len(pool)} matrices "
                     f"(2-D/3-D Laplacians, random symmetric 8-32 nnz/row), n log-uniform in "
                     f"[2^{args.kb_log2n[0]:g}, 2^{args.kb_log2n[1]:g}], Mdeg {mdeg}, {nvec} "
@@ -260,8 +262,90 @@ def kpm_batch_bench(args):
         "mixed_vs_fp64_max_abs_moment_diff": float(d.max()),
         "mixed_vs_fp64_max_rel_count_diff": float(np.max(np.abs(res["mixed"][2] - res["fp64"][2])
                                                          / np.maximum(1.0, np.abs(res["fp64"][2])))),
-    }))
+    }
+    if args.kb_slices > 0:
+        rec["slices"] = kb_slice_phase(args, ev, pool, owner, res["fp64"][1])
+    print(json.dumps(rec))
     ev.finish()
+
+
+def kb_slice_phase(args, ev, pool, owner, mus):
+    """Second phase of the batching form: args.kb_slices ChebLanNr slices
+    (tol 1e-8) drawn from the pool, each an interval holding
+    args.kb_slice_eigs eigenvalues by the first phase's FP64 moments of its
+    matrix, Krylov dimension max(5 (count + 2), 300); all submitted at once to
+    a request loop over the resident pool (every running slice of a matrix
+    advances in one blocked pass per Lanczos step), mixed and FP64."""
+    from evsl_amd.evsl import PREC_FP64, PREC_MIXED, Server
+    srng = np.random.default_rng(99)
+    fams = ["2-D Laplacian", "3-D Laplacian", "random symmetric"]
+    sl = []
+    for j in range(args.kb_slices):
+        r = j % len(mus)
+        m = owner[r]
+        A, lo, hi = pool[m]
+        target = int(srng.integers(args.kb_slice_eigs[0], args.kb_slice_eigs[1] + 1))
+        a = lo + (hi - lo) * srng.uniform(0.05, 0.8)
+        bl, bh = a, hi
+        for _ in range(60):
+            mid = 0.5 * (bl + bh)
+            if kpm_count(mus[r], A.n, lo, hi, a, mid) < target:
+                bl = mid
+            else:
+                bh = mid
+        sl.append((m, a, bh, min(A.n, max(5 * (target + 2), 300)), int(srng.integers(1 << 30))))
+    out, counts = {}, {}
+    for name, prec in (("mixed", PREC_MIXED), ("fp64", PREC_FP64)):
+        srv = Server(ev, 200 << 30, prec)
+        t0 = time.perf_counter()
+        mids = [srv.add_matrix(A, lo, hi) for A, lo, hi in pool]
+        assert min(mids) >= 0, ev.last_error()
+        t_up = time.perf_counter() - t0
+        ids = []
+        ev.reset_stats()
+        t0 = time.perf_counter()
+        for m, a, b, maxit, seed in sl:
+            v0 = np.random.default_rng(seed).uniform(-1, 1, pool[m][0].n)
+            ids.append(srv.submit_slice(mids[m], a, b, maxit, 1e-8, v0))
+            assert ids[-1] >= 0, ev.last_error()
+        t_log = time.perf_counter()
+        while True:
+            left = srv.step()
+            assert left >= 0, ev.last_error()
+            if left == 0:
+                break
+            if time.perf_counter() - t_log > 15.0:
+                log(f"[bench] kpm batch slices {name}: {time.perf_counter() - t0:.0f}s, "
+                    f"{left} left")
+                t_log = time.perf_counter()
+        wall = time.perf_counter() - t0
+        cnt, pairs, rmax = [], 0, 0.0
+        for (m, *_), rid in zip(sl, ids):
+            err, lam, Y, rs = srv.slice_result(rid, pool[m][0].n)
+            assert err == 0
+            cnt.append(len(lam))
+            pairs += len(lam)
+            if len(rs):
+                rmax = max(rmax, float(np.max(rs)))
+        srv.close()
+        counts[name] = cnt
+        st = ev.stats()
+        out[name] = {"wall_s": wall, "upload_s": t_up, "eigenpairs": pairs,
+                     "eigenpairs_per_s": pairs / wall, "max_residual": rmax,
+                     "breakdown_s": {"filter": st["t_filter"], "reorth": st["t_reorth"],
+                                     "ritz_refine": st["t_ritz"]},
+                     "lanczos_steps": int(st["n_lanczos"]),
+                     "refine_columns": int(st["n_refine_cols"]),
+                     "refine_extra_passes": int(st["n_refine_extra"])}
+        log(f"[bench] kpm batch slices {name}: {out[name]}")
+    diff = [{"matrix": sl[i][0], "family": fams[sl[i][0] % 3], "n": int(pool[sl[i][0]][0].n),
+             "mixed": counts["mixed"][i], "fp64": counts["fp64"][i]}
+            for i in range(len(sl)) if counts["mixed"][i] != counts["fp64"][i]]
+    return {"slices": len(sl), "eigs_per_slice": list(args.kb_slice_eigs), "tol": 1e-8,
+            "mixed": out["mixed"], "fp64": out["fp64"],
+            "speedup_eigenpairs_per_s": out["mixed"]["eigenpairs_per_s"] /
+                                        out["fp64"]["eigenpairs_per_s"],
+            "counts_equal": counts["mixed"] == counts["fp64"], "differing_slices": diff}
 
 
 def kpm_count(mu, n, lo, hi, a, b):
@@ -289,6 +373,8 @@ def serve_bench(args):
     rng = np.random.default_rng(777)
     ev = EVSL()
     assert ev.start() == 0
+    if args.no_tune:
+        ev.set_spmm_tuning(False)
     pool = []
     for i in range(args.sv_matrices):
         n = int(np.exp(rng.uniform(np.log(args.sv_n[0]), np.log(args.sv_n[1]))))
@@ -389,6 +475,7 @@ def serve_bench(args):
                     f"1e-6/1e-8; 30% DOS Mdeg 100-400) arriving as a Poisson stream of "
                     f"{args.sv_rate:g}/s over {len(pool)} resident matrices (2-D/3-D "
                     f"Laplacians, random symmetric), n in [{args.sv_n[0]:g}, {args.sv_n[1]:g}]",
+        "spmm_tuning": not args.no_tune,
         "mixed": out["mixed"], "fp64": out["fp64"],
         "speedup_eigenpairs_per_s": out["mixed"]["eigenpairs_per_s"] /
                                     out["fp64"]["eigenpairs_per_s"],
@@ -420,6 +507,8 @@ def c3_bench(args):
     lo, hi = M.gershgorin(A)
     ev = EVSL()
     assert ev.start() == 0
+    if args.no_tune:
+        ev.set_spmm_tuning(False)
     assert ev.set_matrix(A) == 0
     intv = np.array([-0.05, 0.05, lo, hi])
     ev.set_precision(PREC_MIXED if args.form == "mixed" else PREC_FP64)
@@ -503,6 +592,10 @@ def main():
     p.add_argument("--kb-matrices", type=int, default=32)
     p.add_argument("--kb-log2n", type=float, nargs=2, default=[14.0, 20.0],
                    help="matrix sizes log-uniform in [2^lo, 2^hi]")
+    p.add_argument("--kb-slices", type=int, default=64,
+                   help="second phase: slices drawn from the pool (0: none)")
+    p.add_argument("--kb-slice-eigs", type=int, nargs=2, default=[20, 60],
+                   help="eigenvalues per slice of the second phase (range)")
     p.add_argument("--c3", action="store_true",
                    help="run the precision-form workload instead (3-D Anderson 80^3, 16 slices "
                         "of [-0.05, 0.05]) in --form mixed or batch64, dumping eigenvalues "
@@ -514,6 +607,9 @@ def main():
     p.add_argument("--sv-matrices", type=int, default=12)
     p.add_argument("--sv-rate", type=float, default=200.0, help="mean arrivals per second")
     p.add_argument("--sv-n", type=float, nargs=2, default=[1e4, 1e5])
+    p.add_argument("--no-tune", action="store_true",
+                   help="run the fused filter kernel on its fixed layout policy (no per-matrix "
+                        "layout tuning)")
     args = p.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
@@ -535,6 +631,8 @@ def main():
     A, lo, hi = build_problem(args.nx, seed=2024)
     ev = EVSL()
     assert ev.start() == 0
+    if args.no_tune:
+        ev.set_spmm_tuning(False)
     assert ev.set_matrix(A) == 0, ev.last_error()
     cc, dd = 0.5 * (lo + hi), 0.5 * (hi - lo)
     if args.filter_only:
