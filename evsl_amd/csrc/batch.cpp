@@ -786,7 +786,14 @@ int LanczosGroup::add(const double* intv, int maxit, double tol, const double* v
   sl.tol = tol;
   sl.v0 = v0;
   sl.out = out;
-  if (int e = g.start(s, g.mixed && pol->deg >= c.mixed_min_deg && pol->deg <= c.mixed_max_deg))
+  // FP32 filter only for degrees in the mixed range, and only for a filter
+  // that met find_pol's damping thresholds: one that ran out of degrees
+  // (deg = max_deg - 1, SRC/chebpoly.c:363-490) separates its interval
+  // weakly, its slice tends to end at maxit, and there the FP32-filtered
+  // basis leaves pairs just above tol and the slice would be redone in FP64
+  const bool capped = pol->max_deg > 0 && pol->deg >= pol->max_deg - 1;
+  if (int e = g.start(s, g.mixed && !capped && pol->deg >= c.mixed_min_deg &&
+                             pol->deg <= c.mixed_max_deg))
     return e;
   return g.ensure(sl.maxit);
 }

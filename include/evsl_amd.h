@@ -54,7 +54,10 @@ int evsl_amd_set_refine_guard(double guard);
  * filter in FP32 (default [32, 3000]); the others run in FP64 -- a
  * low-degree filter carries few of the bytes and separates the refinement's
  * guard set too weakly, and at very high degrees the refinement's FP32
- * residual recurrence accumulates too much rounding.  A mixed slice whose
+ * residual recurrence accumulates too much rounding.  A filter that did not
+ * meet find_pol's thresholds (deg = max_deg - 1) also runs in FP64: it
+ * separates its interval weakly and its slice tends to end at maxit, where
+ * the FP32-filtered basis leaves pairs just above tol.  A mixed slice whose
  * refinement stalls above tol after its extra passes is redone from its
  * start vector in FP64 (counted in evslAmdStats.n_refine_extra). */
 int evsl_amd_set_mixed_degree_range(int min_deg, int max_deg);
