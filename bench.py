@@ -637,7 +637,12 @@ def main():
     cc, dd = 0.5 * (lo + hi), 0.5 * (hi - lo)
     if args.filter_only:
         # the dominant kernels at their solver widths, then a 512 MiB copy of
-        # known size that calibrates the hardware byte counters
+        # known size that calibrates the hardware byte counters.  The fixed
+        # layout policy runs: under counter collection kernel times are not
+        # representative, so the layout tuner would choose on noise; the
+        # tuned layouts of these keys on this matrix differ from the table
+        # only in rows per lane group (the same kernel instance)
+        ev.set_spmm_tuning(False)
         prec = PREC_MIXED if args.form == "mixed" else PREC_FP64
         B = args.nslices if args.form != "plain" else 1
         out = {}

@@ -107,7 +107,8 @@ hipError_t cheb_spmm_layout(int layout, const Csr& A, int B, bool fp32, Mode mod
 // events (the ChebAv/residual launches with yflush 0 or 3, the steady state
 // of a long series); once every candidate holds its samples and the events
 // have completed, the key keeps the candidate of least time per series step
-// (two yflush-0 launches and one yflush-3 launch).  Layouts give the same
+// (two yflush-0 launches and one yflush-3 launch), if it beats the fixed
+// policy's layout by 3% (near ties keep the table).  Layouts give the same
 // bits, so tuning never changes a result.
 SpmmTuner* spmm_tuner_create();
 void spmm_tuner_destroy(SpmmTuner* t);

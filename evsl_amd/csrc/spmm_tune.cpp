@@ -7,10 +7,10 @@
 // solver's own launches: while a (precision, mode, width) key is undecided,
 // a launch runs one candidate between two timing events; once every
 // candidate holds its samples and the last event has completed (polled, the
-// host never waits for it), the key keeps the fastest candidate.
+// host never waits for it), the key keeps the fastest candidate, or the fixed
+// policy's layout when no candidate beats it by 3%.
 #include <hip/hip_runtime_api.h>
 
-#include <atomic>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -54,9 +54,14 @@ void drop_samples(SpmmTuner::Key& k) {
   k.samples.shrink_to_fit();
 }
 
+// a candidate replaces the table's layout only when it is faster by this
+// fraction: near ties (timing noise is ~1-2%) keep the fixed policy, so the
+// choice is reproducible where the table is already right
+constexpr double kMargin = 0.03;
+
 // the candidate of least time per series step (2 t(yflush 0) + t(yflush 3),
-// each the fastest of its samples), over the kinds every candidate has;
-// the table's layout if no kind is complete
+// each the fastest of its samples), over the kinds every candidate has; the
+// table's layout if no kind is complete or no candidate beats it by kMargin
 int decide(const SpmmTuner::Key& k, int nc, const int* cand, int table) {
   float best[6][2] = {};
   bool have[6][2] = {};
@@ -73,14 +78,16 @@ int decide(const SpmmTuner::Key& k, int nc, const int* cand, int table) {
     for (int cl = 0; cl < 2; ++cl) all[cl] = all[cl] && have[c][cl];
   if (!all[0] && !all[1]) return table;
   int bi = -1;
-  double bt = 0.0;
+  double bt = 0.0, tt = -1.0;
   for (int c = 0; c < nc; ++c) {
     const double t = (all[0] ? 2.0 * best[c][0] : 0.0) + (all[1] ? best[c][1] : 0.0);
+    if (cand[c] == table) tt = t;
     if (bi < 0 || t < bt) {
       bi = c;
       bt = t;
     }
   }
+  if (tt >= 0.0 && bt > (1.0 - kMargin) * tt) return table;
   return cand[bi];
 }
 
