@@ -995,10 +995,13 @@ int LanczosGroup::step() {
     const int nconv = nconvs[j];
     // FP64: the reference's test.  Mixed: the filtered operator is known to
     // ~1e-6, so the trace is compared at that level and the count of Ritz
-    // values above bar must also be unchanged.
+    // values above bar must also be unchanged; both keep the reference's stop
+    // when no Ritz value has reached bar at this test and the one before
+    // (an empty trace, SRC/cheblanNr.c:269-309)
     const double dtr = std::fabs(tr1 - L.tr0);
-    bool done = sl.f32 ? (dtr < 1e-7 * std::max(1, nconv) && nconv == L.last_nconv)
-                       : (dtr < 2e-12 || std::fabs(tr1) + std::fabs(L.tr0) < 1e-10);
+    const bool empty = std::fabs(tr1) + std::fabs(L.tr0) < 1e-10;
+    bool done = sl.f32 ? ((dtr < 1e-7 * std::max(1, nconv) && nconv == L.last_nconv) || empty)
+                       : (dtr < 2e-12 || empty);
     if (fstats)
       std::fprintf(fstats, "slice %d k %4d: nconv %4d  tr1 %21.15e dtr %.3e\n", s, k, nconv, tr1,
                    dtr);
