@@ -1,13 +1,16 @@
 // kernels.hip -- CDNA4 (gfx950) kernels of the filtered-Lanczos / KPM path.
 //
 // Design notes (wave64, 256-thread workgroups):
-//  * cheb_spmm: CSR row blocks are staged through LDS with coalesced loads of
-//    ja/a, then each (row, column) lane walks its row in LDS and gathers the
-//    B contiguous values of x it needs.  The Chebyshev three-term recurrence
-//    (and either the ChebAv accumulation or the KPM moment dot) is fused into
-//    the epilogue, so A is read once per degree for all B columns.  Workgroup
-//    ids are remapped so that each of the 8 XCDs sweeps a contiguous range of
-//    rows (its own L2 then holds the neighbouring x segments it re-reads).
+//  * cheb_spmm (kernel in spmm_kernel.h; fixed-policy launches and the
+//    grouped KPM launch here, the tuner's candidate layouts in
+//    spmm_layouts.hip): CSR row blocks are staged through LDS with coalesced
+//    loads of ja/a, then each (row, column) lane walks its row in LDS and
+//    gathers the B contiguous values of x it needs.  The Chebyshev three-term
+//    recurrence (and either the ChebAv accumulation or the KPM moment dot) is
+//    fused into the epilogue, so A is read once per degree for all B columns.
+//    Workgroup ids are remapped so that each of the 8 XCDs sweeps a
+//    contiguous range of rows (its own L2 then holds the neighbouring x
+//    segments it re-reads).
 //  * Reductions are two-level and deterministic: per-block partials, then a
 //    separate small launch sums them in a fixed order.
 //  * Tall-skinny Gram-Schmidt: V^T w by column groups of 8 with per-chunk

@@ -1,9 +1,10 @@
 // spmm_kernel.h -- the fused Chebyshev-step CSR SpMM kernel (device code),
 // its explicit-layout launcher and the fixed layout policy.  Included by the
 // HIP translation units only: kernels.hip (fixed-policy launches, grouped
-// KPM) and spmm_layouts_f32.hip / spmm_layouts_f64.hip (every candidate
-// layout of the per-matrix layout tuner).  Everything sits in an unnamed
-// namespace, so each unit holds its own instances.
+// KPM) and spmm_layouts.hip, built once per precision (every candidate layout
+// of the per-matrix layout tuner).  The kernel and launchers sit in an
+// unnamed namespace, so each unit holds its own instances; only the two
+// explicit-layout entry points at the end are shared.
 #pragma once
 #include <hip/hip_runtime.h>
 
