@@ -131,6 +131,50 @@ def test_mixed_low_degree_end_interval_matches_fp64(ev):
         ev.free_pol(pol)
 
 
+def test_mixed_capped_filter_runs_fp64(ev):
+    """A filter that ran out of degrees (find_pol's thresholds unmet, deg =
+    max_deg - 1) runs in FP64 under the mixed form even inside the mixed
+    degree range: the result equals the FP64 run bit for bit and no
+    refinement column is spent.  The same interval with room for the degree
+    it needs runs the FP32 filter (refinement columns > 0)."""
+    A = M.perturb_diag(M.laplacian(60, 60, 1), 1e-3, seed=17)
+    lo, hi = M.gershgorin(A)
+    assert ev.set_matrix(A) == 0
+    iv = np.array([0.40, 0.43, lo, hi])
+    v0 = np.random.default_rng(5).uniform(-1, 1, A.n)
+    pols = {}
+    for cap in (40, 3000):
+        pol = ev.default_pol()
+        pol.damping, pol.thresh_int, pol.thresh_ext, pol.max_deg = 2, 0.8, 0.2, cap
+        assert ev.find_pol(iv, pol) == 0
+        pols[cap] = pol
+    assert pols[40].deg == 39 and 32 <= pols[40].deg <= 3000
+    assert 32 <= pols[3000].deg < 2999, pols[3000].deg
+    try:
+        for cap, pol in pols.items():
+            ev.set_precision(PREC_FP64)
+            e64, (r64,) = ev.ChebLanNr_batch([iv], 400, 1e-8, v0, [pol])
+            assert e64 == 0, ev.last_error()
+            ev.set_precision(PREC_MIXED)
+            ev.reset_stats()
+            em, (rm,) = ev.ChebLanNr_batch([iv], 400, 1e-8, v0, [pol])
+            assert em == 0, ev.last_error()
+            cols = ev.stats()["n_refine_cols"]
+            print(f"  max_deg {cap}: deg {pol.deg}, FP64 {len(r64[0])} pairs, mixed "
+                  f"{len(rm[0])}, refinement columns {cols}", flush=True)
+            if cap == 40:
+                assert cols == 0
+                assert np.array_equal(rm[0], r64[0]) and np.array_equal(rm[2], r64[2])
+                assert np.array_equal(rm[1], r64[1])
+            else:
+                assert cols > 0
+                assert len(rm[0]) == len(r64[0])
+    finally:
+        ev.set_precision(PREC_FP64)
+        for pol in pols.values():
+            ev.free_pol(pol)
+
+
 def test_degenerate_spectrum_mixed_pairs_are_true_eigenpairs(ev):
     """Unperturbed 3-D Laplacian: eigenvalues of multiplicity up to 6.  A
     single-start-vector Lanczos in FP64 finds one copy of each multiple
