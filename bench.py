@@ -742,12 +742,21 @@ def main():
                              "max_residual": max(s["max_res"] for s in plain["slices"])}
         rec["speedup_vs_fp64_plain"] = value / pv
         rec["counts_match_fp64_plain"] = pc == rec["per_slice_counts"]
-        diffs = []
+        diffs, per_pair = [], []
         for sp, sm in zip(plain["slices"], last["slices"]):
             if len(sp["lam"]) == len(sm["lam"]) and len(sp["lam"]):
                 a, b = np.sort(sp["lam"]), np.sort(sm["lam"])
-                diffs.append(float(np.max(np.abs(a - b) / np.abs(a))))
+                d = np.abs(a - b) / np.abs(a)
+                diffs.append(float(np.max(d)))
+                per_pair.append(d)
         rec["max_rel_eig_diff_vs_fp64_plain"] = max(diffs) if diffs else None
+        if per_pair:
+            # the error against the FP64 run per eigenpair, as a distribution
+            d = np.concatenate(per_pair)
+            rec["eig_rel_diff_vs_fp64_plain"] = {
+                "pairs": int(d.size), "p50": float(np.percentile(d, 50)),
+                "p90": float(np.percentile(d, 90)), "p99": float(np.percentile(d, 99)),
+                "max": float(d.max()), "per_slice_max": diffs}
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline sample")
         ref = plain if plain is not None else last
