@@ -740,7 +740,7 @@ def main():
         rec["fp64_plain"] = {"value": pv, "wall_s": plain["wall"], "per_slice_counts": pc,
                              "per_slice_steps": [s["steps"] for s in plain["slices"]],
                              "max_residual": max(s["max_res"] for s in plain["slices"])}
-        rec["speedup_vs_fp64_plain"] = value / pv
+        rec["speedup_vs_fp64_plain"] = value / pv if pv > 0 else None
         rec["counts_match_fp64_plain"] = pc == rec["per_slice_counts"]
         diffs, per_pair = [], []
         for sp, sm in zip(plain["slices"], last["slices"]):
