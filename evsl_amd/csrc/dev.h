@@ -69,17 +69,20 @@ hipError_t cheb_spmm(const Csr& A, int B, bool fp32, Mode mode, const void* xk,
 // R rows per lane group), that the layout tuner chooses among for the ChebAv
 // and residual steps of kTuneWidths columns; 0 ends a row.  Every layout
 // sums each row in the same order, so all give the same bits.
-constexpr int kTuneWidths[4] = {4, 8, 16, 32};
-constexpr int kTuneCand[2][4][6] = {{{11, 12, 14, 21, 22, 0},     // FP64, B = 4
-                                     {11, 12, 21, 22, 24, 0},     // 8
-                                     {11, 12, 21, 22, 24, 0},     // 16
-                                     {11, 12, 21, 22, 24, 0}},    // 32
-                                    {{11, 12, 14, 21, 22, 41},    // FP32, B = 4
-                                     {11, 21, 22, 24, 41, 42},    // 8
-                                     {11, 21, 22, 41, 42, 44},    // 16
-                                     {11, 21, 22, 41, 42, 44}}};  // 32
+constexpr int kNumTuneWidths = 5;
+constexpr int kTuneWidths[kNumTuneWidths] = {2, 4, 8, 16, 32};
+constexpr int kTuneCand[2][kNumTuneWidths][6] = {{{11, 12, 21, 0, 0, 0},      // FP64, B = 2
+                                                  {11, 12, 14, 21, 22, 0},    // 4
+                                                  {11, 12, 21, 22, 24, 0},    // 8
+                                                  {11, 12, 21, 22, 24, 0},    // 16
+                                                  {11, 12, 21, 22, 24, 0}},   // 32
+                                                 {{11, 12, 21, 0, 0, 0},      // FP32, B = 2
+                                                  {11, 12, 14, 21, 22, 41},   // 4
+                                                  {11, 21, 22, 24, 41, 42},   // 8
+                                                  {11, 21, 22, 41, 42, 44},   // 16
+                                                  {11, 21, 22, 41, 42, 44}}}; // 32
 inline int tune_width_index(int B) {
-  return B == 4 ? 0 : B == 8 ? 1 : B == 16 ? 2 : B == 32 ? 3 : -1;
+  return B == 2 ? 0 : B == 4 ? 1 : B == 8 ? 2 : B == 16 ? 3 : B == 32 ? 4 : -1;
 }
 // the tuner's candidates for (B, precision): count, *list (null if none)
 inline int tune_candidates(int B, bool fp32, const int** list) {

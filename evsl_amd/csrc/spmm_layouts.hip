@@ -27,7 +27,7 @@ hipError_t launch_layout(int layout, const Csr& A, int B, const void* xk, const 
                          void* xkp1, void* y, const void* v0, const double* s,
                          const double* mu0, double t, double cc, bool first, int yflush,
                          double* part, hipStream_t st) {
-  if constexpr (WI == 4) {
+  if constexpr (WI == kNumTuneWidths) {
     return hipErrorInvalidValue;
   } else if constexpr (cand_at(WI, CI) == 0) {
     return launch_layout<MODE, WI + 1, 0>(layout, A, B, xk, xkm1, xkp1, y, v0, s, mu0, t, cc,

@@ -33,7 +33,7 @@ struct SpmmTuner {
     std::vector<Sample> samples;
   };
   std::mutex mu;
-  Key key[2][2][4];  // [fp32][0 ChebAv, 1 residual][width index]
+  Key key[2][2][kNumTuneWidths];  // [fp32][0 ChebAv, 1 residual][width index]
 };
 
 namespace {

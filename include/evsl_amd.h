@@ -166,7 +166,7 @@ int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, doub
 /* Lane layout of the fused filter kernel (no counterpart in the reference;
  * its CSR matvec has no layout).  A layout is 10*V + R: V adjacent columns
  * per lane and R rows per lane group.  The ChebAv (mode 1) and residual
- * (mode 3) steps of 4, 8, 16 and 32 columns choose theirs per matrix: each
+ * (mode 3) steps of 2, 4, 8, 16 and 32 columns choose theirs per matrix: each
  * device matrix carries a tuner that times the candidate layouts on the
  * solver's own first launches of a (B, prec, mode) key, between HIP events it
  * polls without waiting, and then keeps the fastest.  Every layout sums each

@@ -4,7 +4,7 @@
 Bars:
 * every candidate layout of every tuned key gives the fixed policy's bits:
   the ChebAv step through evsl_amd_cheb_av_cols (the block form of ChebAv,
-  SRC/chebpoly.c:519-598) at 4, 8, 16 and 32 columns in FP64 and in the mixed
+  SRC/chebpoly.c:519-598) at 2, 4, 8, 16 and 32 columns in FP64 and in the mixed
   form, with 16-bit and 32-bit column indices; the residual step through the
   mixed batched solver's refinement with every residual key forced to each
   candidate in turn;
@@ -21,11 +21,12 @@ from evsl_amd.evsl import PREC_FP64, PREC_MIXED
 pytestmark = pytest.mark.gpu
 
 ERR_ARG = -104
-WIDTHS = (4, 8, 16, 32)
+WIDTHS = (2, 4, 8, 16, 32)
 # candidates (10*V + R) per precision and width, as the kernel library lists them
 CANDIDATES = {
-    PREC_FP64: {B: [11, 12, 14, 21, 22] if B == 4 else [11, 12, 21, 22, 24] for B in WIDTHS},
-    PREC_MIXED: {4: [11, 12, 14, 21, 22, 41], 8: [11, 21, 22, 24, 41, 42],
+    PREC_FP64: {2: [11, 12, 21], 4: [11, 12, 14, 21, 22], 8: [11, 12, 21, 22, 24],
+                16: [11, 12, 21, 22, 24], 32: [11, 12, 21, 22, 24]},
+    PREC_MIXED: {2: [11, 12, 21], 4: [11, 12, 14, 21, 22, 41], 8: [11, 21, 22, 24, 41, 42],
                  16: [11, 21, 22, 41, 42, 44], 32: [11, 21, 22, 41, 42, 44]},
 }
 
@@ -59,7 +60,7 @@ def test_tuner_candidates_and_argument_errors(ev):
                 assert ev.set_spmm_layout(B, prec, mode, 0) == 0
                 lay, dec = ev.spmm_layout(B, prec, mode)
                 assert dec and lay in ok  # the fixed policy is one of the candidates
-    for B in (1, 2, 3, 6, 7):
+    for B in (1, 3, 6, 7):
         assert ev.set_spmm_layout(B, PREC_FP64, 1, 11) == ERR_ARG
     assert ev.set_spmm_layout(8, PREC_FP64, 2, 11) == ERR_ARG  # KPM keeps its layout
     assert ev.set_spmm_layout(8, PREC_FP64, 1, -2) == ERR_ARG
