@@ -43,6 +43,11 @@ __device__ __forceinline__ double block_sum(double v, double* sh /*>=4*/) {
   return r;
 }
 
+// KPM steps of several matrices in one launch: global row block gb (after
+// the XCD remap) belongs to matrix owner[gb], whose entry names its CSR, its
+// row-block table, its vector blocks and its recurrence factors; step k
+// reads T_k and T_{k-1} from the matrix's ring and writes T_{k+1} and the
+// block's moment partials, as the one-matrix kKpm launch does.
 template <typename VT, int B, int V>
 __global__ __launch_bounds__(kWG) void k_kpm_group(const KpmGroupEntry* __restrict__ tab,
                                                    const int* __restrict__ owner, int total,

@@ -331,11 +331,6 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
                                                 first, yflush, part, stage_ia ? s_dyn : nullptr);
 }
 
-// KPM steps of several matrices in one launch: global row block gb (after
-// the XCD remap) belongs to matrix owner[gb], whose entry names its CSR, its
-// row-block table, its vector blocks and its recurrence factors; step k
-// reads T_k and T_{k-1} from the matrix's ring and writes T_{k+1} and the
-// block's moment partials, as the one-matrix kKpm launch does.
 // Non-temporal streams (the NT instances) exist for the ChebAv and residual
 // steps of 8 or more columns.  They are chosen when one launch's working set
 // -- the matrix, and the vector blocks xk, xkm1, xkp1, y (and R) -- exceeds
