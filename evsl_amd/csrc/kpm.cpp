@@ -198,7 +198,10 @@ int spslicer(double* sli, double* mu, int Mdeg, double* intv, int n_int, int npt
   while (++ls < n_int) {
     while (ii < npts && yi[ii] < target) ii++;
     if (ii == npts) break;
-    if ((target - yi[ii - 1]) < (yi[ii] - target)) ii--;
+    // the nearer of the two grid points; the reference reads yi[-1] when
+    // target <= yi[0] (a density whose integral is not positive): the first
+    // grid point has no lower neighbour and is kept
+    if (ii > 0 && (target - yi[ii - 1]) < (yi[ii] - target)) ii--;
     sli[ls] = ctr + wid * xi[ii];
     target = yi[ii] + (yi[npts - 1] - yi[ii]) / (n_int - ls);
   }

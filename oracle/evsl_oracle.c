@@ -15,17 +15,31 @@
  *   ora_ChebAv        dense eigendecomposition: y = Q p(L) Q^T v
  *   ora_kpmdos        dense eigendecomposition of the same random vectors
  *   ora_intChx        numerical quadrature of the KPM density
- *   ora_spslicer      grid-point / equal-share invariants
+ *   ora_spslicer      closed-form boundaries of a density uniform in theta
+ *                     (exact grid indices from arccos), invariants
  *   ora_CGS_DGKS      (I - Q Q^T) v and orthogonality
  *   ora_tridiag_eig   LAPACK (scipy.linalg.eigh_tridiagonal)
  *   ora_sym_eig       LAPACK (numpy.linalg.eigh)
  *   ora_ChebLanNr     closed-form Laplacian spectra and dense eigh counts
  *   ora_ChebLanTr     closed-form Laplacian spectra and dense eigh counts
- *   ora_LanBounds     enclosure of the dense spectrum; exact with full Krylov
- *   ora_LanDos        exact eigenvalue counts (statistical bound)
+ *   ora_LanBounds     Ritz values and residual norms of an independently
+ *                     built (QR) Krylov space: lmin = theta_1 - ||r_1||
+ *   ora_LanDos        Gauss quadrature of a full Krylov space: the smoothed
+ *                     DOS in closed form from the eigenpairs of A (1e-12)
  *   ora_simpson       exact for cubics
- *   ora_spslicer2     equal-share invariants
+ *   ora_spslicer2     closed-form boundaries of linear and constant densities
+ *
+ * ORA_PLANT (default 0) compiles one planted mistake into a copy of this
+ * file, for the tests that show each pin above fails on a plausible error
+ * (tests/test_oracle_pins.py builds those copies in a temporary directory):
+ *   1 spslicer without the nearest-point snap   2 spslicer with fixed shares
+ *   3 spslicer2 advancing t by the share        4 LanBounds without |beta s|
+ *   5 LanBounds with the first Ritz-vector row  6 LanDos weights from the last row
+ *   7 LanDos scaled without 1/nvec             8 LanDos with M = msteps (no cap)
  */
+#ifndef ORA_PLANT
+#define ORA_PLANT 0
+#endif
 #define _POSIX_C_SOURCE 199309L
 #include "evsl_oracle.h"
 
@@ -516,9 +530,15 @@ int ora_spslicer(double *sli, const double *mu, int Mdeg, const double *intv,
   while (++ls < n_int) {
     while (ii < npts && yi[ii] < target) ii++;
     if (ii == npts) break;
-    if ((target - yi[ii - 1]) < (yi[ii] - target)) ii--;
+    /* the nearer of the two grid points; the reference reads yi[-1] when
+     * target <= yi[0] (a density whose integral is not positive), here the
+     * first grid point has no lower neighbour and is kept */
+    if (ORA_PLANT != 1 && ii > 0 && (target - yi[ii - 1]) < (yi[ii] - target)) ii--;
     sli[ls] = ctr + wid * xi[ii];
-    target = yi[ii] + (yi[npts - 1] - yi[ii]) / (n_int - ls);
+    if (ORA_PLANT == 2)
+      target = yi[npts - 1] * (ls + 1) / (double)n_int;
+    else
+      target = yi[ii] + (yi[npts - 1] - yi[ii]) / (n_int - ls);
   }
   sli[n_int] = intv[1];
   if (ls != n_int) err = 1;
@@ -1071,8 +1091,16 @@ int ora_LanBounds(int msteps, const double *v, double *lmin, double *lmax) {
   size_t ml = msteps;
   double *S = malloc(ml * ml * sizeof(double)), *ritz = malloc(msteps * sizeof(double));
   int info = ora_tridiag_eig(msteps, alp, bet, ritz, S);
-  *lmin = ritz[0] - fabs(bottomBeta * S[msteps - 1]);
-  *lmax = ritz[msteps - 1] + fabs(bottomBeta * S[ml * ml - 1]);
+  if (ORA_PLANT == 4) {
+    *lmin = ritz[0];
+    *lmax = ritz[msteps - 1];
+  } else if (ORA_PLANT == 5) {
+    *lmin = ritz[0] - fabs(bottomBeta * S[0]);
+    *lmax = ritz[msteps - 1] + fabs(bottomBeta * S[ml * ml - ml]);
+  } else {
+    *lmin = ritz[0] - fabs(bottomBeta * S[msteps - 1]);
+    *lmax = ritz[msteps - 1] + fabs(bottomBeta * S[ml * ml - 1]);
+  }
   free(alp);
   free(bet);
   free(V);
@@ -1377,7 +1405,7 @@ void ora_spslicer2(const double *xi, double *yi, int n_int, int npts, double *sl
     if (yi[k] - t >= want) {
       ls = ls + 1;
       sli[ls] = xi[k];
-      t = yi[k];
+      t = ORA_PLANT == 3 ? t + want : yi[k];
     }
   }
   sli[n_int] = xi[npts - 1];
@@ -1396,7 +1424,7 @@ int ora_LanDos(int nvec, int msteps, int npts, double *xdos, double *ydos, doubl
   const double tolBdwn = 1.e-13 * (fabs(lM) + fabs(lm));
   const double aa = evsl_max(intv[0], intv[2]), bb = evsl_min(intv[1], intv[3]);
   const double kappa = 1.25;
-  const int M = evsl_min(msteps, 30);
+  const int M = ORA_PLANT == 8 ? msteps : evsl_min(msteps, 30);
   const double H = (lM - lm) / (M - 1);
   const double sigma = H / sqrt(8 * log(kappa));
   double sigma2 = 2 * sigma * sigma;
@@ -1447,7 +1475,10 @@ int ora_LanDos(int nvec, int msteps, int npts, double *xdos, double *ydos, doubl
       }
     }
     if (ora_tridiag_eig(msteps, alp, bet, ritz, S)) return -3;
-    for (int i = 0; i < msteps; i++) gamma2[i] = S[i * ml0] * S[i * ml0];
+    for (int i = 0; i < msteps; i++) {
+      const size_t at = ORA_PLANT == 6 ? i * ml0 + msteps - 1 : i * ml0;
+      gamma2[i] = S[at] * S[at];
+    }
     for (int i = 0; i < msteps; i++) {
       const double tt = ritz[i];
       int numPlaced = 0;
@@ -1457,7 +1488,7 @@ int ora_LanDos(int nvec, int msteps, int npts, double *xdos, double *ydos, doubl
         y[ind[j]] += gamma2[i] * exp(-((xdos[ind[j]] - tt) * (xdos[ind[j]] - tt)) / sigma2);
     }
   }
-  double scaling = 1.0 / (nvec * sqrt(sigma2 * EVSL_PI));
+  double scaling = 1.0 / ((ORA_PLANT == 7 ? 1 : nvec) * sqrt(sigma2 * EVSL_PI));
   scal(npts, scaling, y);
   memcpy(ydos, y, npts * sizeof(double));
   ora_simpson(xdos, y, npts);
