@@ -19,6 +19,8 @@ namespace evsl_dev {
 constexpr int kWG = 256;
 constexpr int kNnzCap = 2048;
 constexpr int kRowsCap = 256;
+// a row longer than kNnzCap is summed in this many fixed stripes
+constexpr int kLongStripes = 32;
 constexpr int kNumWidths = 10;  // B = 1..8, 16, 32
 inline int width_slot(int B) { return B <= 8 ? B - 1 : (B == 16 ? 8 : (B == 32 ? 9 : -1)); }
 inline int width_of_slot(int s) { return s < 8 ? s + 1 : (s == 8 ? 16 : 32); }

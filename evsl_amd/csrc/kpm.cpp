@@ -237,13 +237,51 @@ void spslicer2(double* xi, double* yi, int n_int, int npts, double* sli) {
   int ls = 0;
   sli[ls] = xi[0];
   for (int k = 1; k < npts; k++) {
-    if (yi[k] - t >= want) {
+    // (ls < n_int: a further boundary would fall past sli[n_int], which the
+    // reference writes out of bounds when rounding lets one more share fit)
+    if (yi[k] - t >= want && ls < n_int) {
       ls = ls + 1;
       sli[ls] = xi[k];
       t = yi[k];
     }
   }
   sli[n_int] = xi[npts - 1];
+}
+
+// The exact smoothed DOS of known eigenvalues (SRC/exDOS.c:17-69): Gaussians
+// of width sigma = H / sqrt(8 ln 1.25), H = (lmax - lmin) / (min(n, 30) - 1),
+// cut where they fall below 1e-8, on npts points of [a, b], scaled by
+// 1 / (n sqrt(2 pi sigma^2)).  The reference walks each Gaussian's points by
+// repeated addition of the grid step (xi += h) from its first point; the same
+// walk here keeps its values.  Returns 1 when the last eigenvalue's walk left
+// the point counter at 0 (in particular when no eigenvalue lies in [a, b]),
+// else 0.  Host-only.
+int exDOS(double* vals, int n, int npts, double* x, double* y, double* intv) {
+  const double lm = intv[2], lM = intv[3], a = intv[0], b = intv[1];
+  const int M = std::min(n, 30);
+  const double H = (lM - lm) / (M - 1);
+  const double sigma = H / std::sqrt(8 * std::log(1.25));
+  const double sigma2 = 2 * sigma * sigma;
+  const double width = sigma * std::sqrt(-2.0 * std::log(1e-08));
+  linspace(a, b, npts, x);
+  const double h = x[1] - x[0];
+  std::memset(y, 0, (size_t)npts * sizeof(double));
+  int j = 0;
+  for (int i = 0; i < n; i++) {
+    const double t = vals[i];
+    if (t < a || t > b) continue;
+    j = std::max((int)((t - a - width) / h), 0);
+    const double stop = std::min(t + width, b);
+    // j < npts: the reference relies on the accumulated xi passing b first
+    for (double xi = a + j * h; xi <= stop && j < npts; xi += h) {
+      y[j] += std::exp(-(xi - t) * (xi - t) / sigma2);
+      j++;
+    }
+  }
+  if (j == 0) return 1;
+  const double scaling = 1.0 / (n * std::sqrt(sigma2 * EVSL_PI));
+  for (int k = 0; k < npts; k++) y[k] *= scaling;
+  return 0;
 }
 
 // Time nsteps fused filter launches (block width B, prec) on the current

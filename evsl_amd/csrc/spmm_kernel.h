@@ -79,7 +79,9 @@ __device__ __forceinline__ void cheb_spmm_block(
     jx = ja;
   constexpr size_t kStage = LDS ? (size_t)kNnzCap * (sizeof(IT) + sizeof(VT)) : 0;
   constexpr size_t kRed = (size_t)kWG * V * sizeof(double);
-  __shared__ __align__(16) unsigned char smem[kStage > kRed ? kStage : kRed];
+  constexpr size_t kLong = (size_t)kLongStripes * B * sizeof(double);
+  constexpr size_t kSmem0 = kStage > kRed ? kStage : kRed;
+  __shared__ __align__(16) unsigned char smem[kSmem0 > kLong ? kSmem0 : kLong];
   IT* s_ja = reinterpret_cast<IT*>(smem);
   VT* s_a = reinterpret_cast<VT*>(smem + (size_t)kNnzCap * sizeof(IT));
   double* s_red = reinterpret_cast<double*>(smem);
@@ -270,18 +272,25 @@ __device__ __forceinline__ void cheb_spmm_block(
     }
     if (MODE == kKpm && LDS) __syncthreads();  // s_red aliases the staging area
   } else {
-    // one long row: all groups split its nonzeros, combine through LDS
-    VT acc[V];
+    // one long row, split into kLongStripes fixed stripes (stripe s sums the
+    // nonzeros p0 + s, p0 + s + kLongStripes, ... in order), whose sums are
+    // then added in stripe order in double: the same arithmetic per column
+    // for every lane layout and block width, so tuning and batching keep
+    // their bits on such rows too.  Lane group g takes stripes g, g + NG, ...
+    double* s_str = reinterpret_cast<double*>(smem);  // kLongStripes x B
+    for (int st = grp; act && st < kLongStripes; st += NG) {
+      VT acc[V];
 #pragma unroll
-    for (int j = 0; j < V; ++j) acc[j] = 0;
-    for (int p = p0 + grp; act && p < p1; p += NG) {
-      const VT a = av[p];
-      const Vec x = ld(xk, (size_t)((I16 ? r0 : 0) + (int)jx[p]) * B + c0);
+      for (int j = 0; j < V; ++j) acc[j] = 0;
+      for (int p = p0 + st; p < p1; p += kLongStripes) {
+        const VT a = av[p];
+        const Vec x = ld(xk, (size_t)((I16 ? r0 : 0) + (int)jx[p]) * B + c0);
 #pragma unroll
-      for (int j = 0; j < V; ++j) acc[j] += a * x.v[j];
+        for (int j = 0; j < V; ++j) acc[j] += a * x.v[j];
+      }
+#pragma unroll
+      for (int j = 0; j < V; ++j) s_str[st * B + c0 + j] = (double)acc[j];
     }
-#pragma unroll
-    for (int j = 0; j < V; ++j) s_red[tid * V + j] = (double)acc[j];
     __syncthreads();
     if (act && grp == 0) {
       VT tot[V];
@@ -289,7 +298,7 @@ __device__ __forceinline__ void cheb_spmm_block(
       for (int j = 0; j < V; ++j) {
         double sj = 0.0;
 #pragma unroll 1
-        for (int gg = 0; gg < NG; ++gg) sj += s_red[(gg * L + sub) * V + j];
+        for (int st = 0; st < kLongStripes; ++st) sj += s_str[st * B + c0 + j];
         tot[j] = (VT)sj;
       }
       epilogue(r0, tot, prefetch(r0));

@@ -140,6 +140,9 @@ int spslicer(double *sli, double *mu, int Mdeg, double *intv, int n_int, int npt
 int LanDos(const int nvec, int msteps, int npts, double *xdos, double *ydos,
            double *neig, const double *const intv);
 void spslicer2(double *xi, double *yi, int n_int, int npts, double *sli);
+/* exact smoothed DOS of known eigenvalues (INC/evsl.h:180, SRC/exDOS.c:17-69;
+ * the reference LanDos driver's comparison curve), host only */
+int exDOS(double *vals, int n, int npts, double *x, double *y, double *intv);
 
 /* solvers (INC/evsl.h:142-145, 183) */
 int ChebLanNr(double *intv, int maxit, double tol, double *vinit, polparams *pol,

@@ -167,3 +167,54 @@ def test_tuner_settles_and_measuring_calls_keep_bits(ev, prec):
     err, Y = ev.cheb_av_cols([pol] * 8, X, prec)
     assert err == 0 and np.array_equal(Y, ref)
     ev.free_pol(pol)
+
+
+def arrow_matrix(n, dense_rows, seed):
+    """A random symmetric sparse matrix plus `dense_rows` rows (and columns)
+    that touch every column: rows far longer than one row block's 2048
+    nonzeros, which the filter kernel splits over the lanes of a block."""
+    import scipy.sparse as sp
+    from evsl_amd.types import HostCsr
+    base = M.random_symmetric(n, 4, 8, seed=seed).to_scipy()
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(np.arange(dense_rows), n)
+    cols = np.tile(np.arange(n), dense_rows)
+    vals = rng.standard_normal(rows.size) * 0.05
+    D = sp.coo_matrix((vals, (rows, cols)), shape=(n, n)).tocsr()
+    A = (base + D + D.T).tocsr()
+    A.sum_duplicates()
+    A.sort_indices()
+    return HostCsr.from_scipy(A)
+
+
+@pytest.mark.parametrize("n", [6000, 40000])
+def test_long_rows_same_bits_every_layout_and_width(ev, n):
+    """Rows of n nonzeros (longer than a row block's 2048; 16-bit column
+    offsets at n = 6000, 32-bit indices at 40000): every width and every
+    candidate layout gives each column the bits of its one-column run."""
+    A = arrow_matrix(n, 3 if n < 10000 else 2, seed=5)
+    assert np.diff(A.ia).max() > 2 * 2048
+    assert ev.set_matrix(A) == 0, ev.last_error()
+    assert ev.lib.evsl_amd_spmm_index_bytes() == (2 if n < 32768 else 4)
+    lo, hi = M.gershgorin(A)
+    (pol,) = make_pols(ev, [(-0.2, 0.4)], lo, hi)
+    X = np.random.default_rng(8).uniform(-1.0, 1.0, (A.n, 32))
+    for prec in (PREC_FP64, PREC_MIXED):
+        ev.set_spmm_tuning(False)
+        one = np.empty_like(X)
+        for c in range(32):
+            err, Yc = ev.cheb_av_cols([pol], X[:, c:c + 1], prec)
+            assert err == 0, ev.last_error()
+            one[:, c] = Yc[:, 0]
+        for B in (1, 3, 6, 7):
+            err, Y = ev.cheb_av_cols([pol] * B, X[:, :B], prec)
+            assert err == 0 and np.array_equal(Y, one[:, :B]), (prec, B)
+        ev.set_spmm_tuning(True)
+        for B in WIDTHS:
+            for lay in CANDIDATES[prec][B]:
+                assert ev.set_spmm_layout(B, prec, 1, lay) == 0
+                err, Y = ev.cheb_av_cols([pol] * B, X[:, :B], prec)
+                assert err == 0, ev.last_error()
+                assert np.array_equal(Y, one[:, :B]), (prec, B, lay, np.abs(Y - one[:, :B]).max())
+            assert ev.set_spmm_layout(B, prec, 1, -1) == 0
+    ev.free_pol(pol)
