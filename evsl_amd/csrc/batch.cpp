@@ -59,9 +59,8 @@ int block_filter_seg(int B, const FilterOp& op, const double* mus_d, int k0, int
     EVSL_HIP(evsl_dev::cheb_spmm(op.A, B, op.f32, evsl_dev::kCheb, xk, xkm1, xkp1, Y, nullptr,
                                  mus_d + (size_t)k * B, mus_d, k == 1 ? op.t1 : op.t2, op.cc,
                                  k == 1, yf, nullptr, c.stream));
-    const int streams = evsl_dev::cheb_streams(yf, k == 1);
     c.astats.filter_bytes += (double)c.A.nnz * (vs + c.A.idx_bytes()) + 4.0 * (c.n + 1) +
-                             (double)streams * B * c.n * vs;
+                             evsl_dev::cheb_vec_bytes(yf, k == 1, (int)vs) * B * c.n;
   }
   c.astats.n_filter_steps += k1 - k0 + 1;
   c.astats.n_filter_cols += (size_t)(k1 - k0 + 1) * B;
@@ -74,7 +73,8 @@ int block_filter(int B, const FilterOp& op, const double* mus_d, int maxdeg, con
   return block_filter_seg(B, op, mus_d, 1, maxdeg, X0, bufs, Y);
 }
 
-// Shared work buffers of the blocked filter (sized for Bmax columns).  The
+// Shared work buffers of the blocked filter (sized for Bmax columns; the
+// output blocks Y, Y2 are FP64 in both precisions).  The
 // coefficient tables stay on the device while the same filters come back
 // (the batched solver filters the same slices step after step), so a filter
 // pass issues no host-device copy and no synchronisation.
@@ -90,8 +90,8 @@ struct BlockWork {
     const size_t nb = (size_t)n * B * vs;
     EVSL_HIP(X0.alloc(nb));
     EVSL_HIP(bufs.alloc(3 * nb));
-    EVSL_HIP(Y.alloc(nb));
-    EVSL_HIP(Y2.alloc(nb));
+    EVSL_HIP(Y.alloc((size_t)n * B * sizeof(double)));
+    EVSL_HIP(Y2.alloc((size_t)n * B * sizeof(double)));
     EVSL_HIP(zero.alloc((size_t)n * sizeof(double)));
     EVSL_HIP(junk.alloc((size_t)n * sizeof(double)));
     if (B > Bcap) {
@@ -185,7 +185,7 @@ int filter_columns_phased(int b, const double* const* xin, double* const* yout,
     if (err) return;
     const double* mus_d = W.mus.as<double>() + W.mus_off[phase++];
     if ((err = block_filter_seg(active, op, mus_d, k0, kend, W.X0.p, slot, Y))) return;
-    hipError_t he = evsl_dev::unpack_range(n, active, keep, active, Y, f32,
+    hipError_t he = evsl_dev::unpack_range(n, active, keep, active, Y, false,
                                            col_ptrs(out.data() + keep, active - keep), c.stream);
     if (he == hipSuccess && keep > 0) {
       // T_kend sits in slot (kend-1)%3 and T_{kend-1} in slot (kend-2)%3
@@ -199,9 +199,9 @@ int filter_columns_phased(int b, const double* const* xin, double* const* yout,
       slot[sa] = pf;
       slot[sb] = pa;
       slot[sf] = pb;
-      if (he == hipSuccess) he = evsl_dev::narrow_cols(n, active, keep, Y, Yspare, f32, c.stream);
+      if (he == hipSuccess) he = evsl_dev::narrow_cols(n, active, keep, Y, Yspare, false, c.stream);
       std::swap(Y, Yspare);
-      c.astats.filter_bytes += 3.0 * (double)n * (active + keep) * vs;
+      c.astats.filter_bytes += (2.0 * vs + 8.0) * (double)n * (active + keep);
     }
     if (he != hipSuccess) {
       set_error("filter_columns_phased: unpack/narrow", he);
@@ -272,7 +272,7 @@ int filter_columns(int ncols, const double* const* xin, double* const* yout,
     FilterOp op;
     if (int e = make_filter_op(pols[j0]->cc, pols[j0]->dd, f32, &op)) return e;
     if (int e = block_filter(B, op, W.mus.as<double>(), md, W.X0.p, bufs, W.Y.p)) return e;
-    EVSL_HIP(evsl_dev::unpack_cols(n, B, W.Y.p, f32, col_ptrs(out.data(), B), c.stream));
+    EVSL_HIP(evsl_dev::unpack_cols(n, B, W.Y.p, false, col_ptrs(out.data(), B), c.stream));
   }
   return 0;
 }
@@ -357,6 +357,7 @@ struct Slice {
 int residual_filter(int n, int m, double* U, double* W, const polparams* pol, BlockWork& bw) {
   Context& c = ctx();
   const int deg = pol->deg;
+  c.astats.n_refine_degcols += (size_t)m * deg;
   if (deg == 0) {
     for (int j = 0; j < m; ++j) EVSL_HIP(evsl_dev::scal(n, pol->mu[0], U + (size_t)j * n, c.stream));
     return 0;
@@ -446,9 +447,9 @@ int residual_filter(int n, int m, double* U, double* W, const polparams* pol, Bl
                                      op.cc, k == 1, yf, nullptr, c.stream));
         // the kCheb streams plus the residual block R
         c.astats.refine_bytes += (double)c.A.nnz * (4 + c.A.idx_bytes()) + 4.0 * (n + 1) +
-                                 (double)(evsl_dev::cheb_streams(yf, k == 1) + 1) * B * n * 4;
+                                 (evsl_dev::cheb_vec_bytes(yf, k == 1, 4) + 4.0) * B * n;
       }
-      EVSL_HIP(evsl_dev::unpack_cols(n, B, Yb, true, col_ptrs(out.data(), B), c.stream));
+      EVSL_HIP(evsl_dev::unpack_cols(n, B, Yb, false, col_ptrs(out.data(), B), c.stream));
     }
   }
   // U <- U p(theta~) + Y
@@ -536,6 +537,50 @@ int refine_extract(int n, int kdim, const double* dT, const double* eT, const do
     // components outside the guard set by at least the guard ratio.
     constexpr int kMaxExtraPasses = 2;
     constexpr double kRetryFactor = 1e3;
+    // the first pass's polynomial: the interval widened by refine_widen
+    // slice widths on each side, same damping and thresholds (a lower
+    // degree; the FP32-filtered basis carries rounding noise spread over
+    // the whole spectrum, and the residual weighs a component by its
+    // eigenvalue's distance, so the far part is what a pass must damp)
+    polparams qpol = *pol;
+    qpol.mu = nullptr;
+    bool have_q = false;
+    if (c.refine_widen > 0.0) {
+      const double w = bb - aa, lmin = intv[2], lmax = intv[3];
+      double qiv[4] = {std::max(lmin, aa - c.refine_widen * w),
+                       std::min(lmax, bb + c.refine_widen * w), lmin, lmax};
+      qpol.deg = 0;
+      if (find_pol(qiv, &qpol) == 0 && qpol.deg < pol->deg && qpol.cc == pol->cc &&
+          qpol.dd == pol->dd)
+        have_q = true;
+      else
+        free_pol(&qpol);
+    }
+    struct PolGuard {
+      polparams* p;
+      ~PolGuard() { free_pol(p); }
+    } qguard{&qpol};
+    if (fstats) {
+      // residuals of the Ritz vectors as the FP32-filtered basis gives them
+      EVSL_HIP(evsl_dev::col_dot(n, m, Ud, n, Ud, n, part.as<double>(), s2.as<double>(),
+                                 c.stream));
+      if (int err = apply_A_cols(n, m, Ud, Wd, bw)) return err;
+      if (int err = measure()) return err;
+      double rmax = 0.0;
+      int in_ab = 0, below = 0;
+      for (int j = 0; j < m; ++j) {
+        const double t = ht[j] / hs2[j];
+        if (t < aa - kEpsM || t > bb + kEpsM) continue;
+        const double r = std::sqrt(std::max(hr2[j], 0.0) / hs2[j]);
+        ++in_ab;
+        below += r < tol;
+        rmax = std::max(rmax, r);
+      }
+      std::fprintf(fstats,
+                   "  refinement: first pass degree %d (filter %d); before it %d Ritz values "
+                   "in [a,b], %d below tol, max residual %.3e\n",
+                   have_q ? qpol.deg : pol->deg, pol->deg, in_ab, below, rmax);
+    }
     DevBuf G, gp, M, U2;
     EVSL_HIP(G.alloc((size_t)m * m * sizeof(double)));
     EVSL_HIP(gp.alloc(evsl_dev::gemm_tn_part_elems(n, m, m) * sizeof(double)));
@@ -543,7 +588,8 @@ int refine_extract(int n, int kdim, const double* dT, const double* eT, const do
     EVSL_HIP(U2.alloc((size_t)n * m * sizeof(double)));
     const int min_pass = std::max(c.refine_passes, 1);
     for (int pass = 0;; ++pass) {
-      if (int err = residual_filter(n, m, Ud, Wd, pol, bw)) return err;
+      if (int err = residual_filter(n, m, Ud, Wd, pass == 0 && have_q ? &qpol : pol, bw))
+        return err;
       if (pass > 0) c.astats.n_refine_cols += m;
       // Cholesky QR: G = U^T U = R^T R, U <- U R^-1
       for (int qr = 0; qr < 2; ++qr) {
@@ -649,15 +695,15 @@ int refine_extract(int n, int kdim, const double* dT, const double* eT, const do
 // joined.
 //
 // In the mixed form a slice filters in FP32 only when its polynomial degree
-// lies in a range (evsl_amd_set_mixed_degree_range, default [32, 3000]): a
+// lies in a range (evsl_amd_set_mixed_degree_range, default [32, 20000]): a
 // lower-degree filter carries few of the bytes (the re-orthogonalisation
 // dominates) and separates the refinement's guard set too weakly (an end
 // interval's degree-4 filter left every pair at ~2e-7 after three refinement
-// passes); at a much higher degree the FP32 residual recurrence of the
-// refinement accumulates rounding like k^1.5 eps_fp32 relative to the
-// residual (~4% at degree 7250, where 2 of 16 slices stalled).  A mixed slice
-// whose refinement still stalls above tol is run again from its start vector
-// in FP64.
+// passes).  At high degrees (7250 on the Anderson model at band centre) the
+// FP64 sum of the series and the lower-degree first refinement pass keep the
+// refinement's FP32 residual recurrence short and its noise low.  A mixed
+// slice whose refinement still stalls above tol is run again from its start
+// vector in FP64.
 
 struct LanczosGroup::Impl {
   struct Slot {
@@ -792,9 +838,10 @@ int LanczosGroup::add(const double* intv, int maxit, double tol, const double* v
   // weakly, its slice tends to end at maxit, and there the FP32-filtered
   // basis leaves pairs just above tol and the slice would be redone in FP64
   const bool capped = pol->max_deg > 0 && pol->deg >= pol->max_deg - 1;
-  if (int e = g.start(s, g.mixed && !capped && pol->deg >= c.mixed_min_deg &&
-                             pol->deg <= c.mixed_max_deg))
-    return e;
+  const bool f32 =
+      g.mixed && !capped && pol->deg >= c.mixed_min_deg && pol->deg <= c.mixed_max_deg;
+  if (g.mixed && !f32) ++c.astats.n_fp64_slices;
+  if (int e = g.start(s, f32)) return e;
   return g.ensure(sl.maxit);
 }
 
@@ -1024,7 +1071,7 @@ int LanczosGroup::step() {
         std::free(o->res);
         if (o->W) _evsl_Free_device(o->W);
         *o = SliceResult();
-        ++c.astats.n_refine_extra;
+        ++c.astats.n_fp64_reruns;
         if (int e = g.start(s, false)) return e;
         continue;
       }

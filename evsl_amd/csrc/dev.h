@@ -51,7 +51,8 @@ enum Mode { kPlain = 0, kCheb = 1, kKpm = 2, kRes = 3 };
 //   acc = A * xk
 //   kPlain: xkp1 = acc
 //   kCheb : w = t*(acc - cc*xk) - (xkm1 ? xkm1 : 0); xkp1 = w;
-//           y accumulates the last yflush terms of sum_k mu_k T_k:
+//           y (FP64 in both precisions: each term is widened and added in
+//           FP64) accumulates the last yflush terms of sum_k mu_k T_k:
 //             first (yflush 1): y = mu0[c]*xk + s[c]*w
 //             yflush 3: y += s[c-2B]*xkm1, then += s[c-B]*xk, then += s[c]*w
 //             yflush 2: y += s[c-B]*xk, then += s[c]*w;  1: y += s[c]*w
@@ -61,7 +62,8 @@ enum Mode { kPlain = 0, kCheb = 1, kKpm = 2, kRes = 3 };
 //           xkp1 = w; y as kCheb with coefficients mu0[0] (this degree),
 //           mu0[-2], mu0[-4] (the two before) and mu0[1] for xk when first
 // s, mu0: per-column coefficients (device; kRes see above).
-// fp32 selects float vectors and matrix values (float accumulation).
+// fp32 selects float vectors and matrix values (float accumulation); y is
+// always a block of doubles.
 hipError_t cheb_spmm(const Csr& A, int B, bool fp32, Mode mode, const void* xk,
                      const void* xkm1, void* xkp1, void* y, const void* v0,
                      const double* s, const double* mu0, double t, double cc,
@@ -142,10 +144,13 @@ inline int y_flush(int k, int m) {
   return k == m ? r : 0;
 }
 
-// algorithmic vector streams of one kCheb launch (kRes: one more, the
-// residual block): xk (gathered once), xkm1, xkp1, and y in and out when
-// it is flushed; the first launch reads xk and writes xkp1 and y
-inline int cheb_streams(int yflush, bool first) { return first ? 3 : 3 + (yflush ? 2 : 0); }
+// algorithmic vector bytes per row and column of one kCheb launch with
+// vectors of vs bytes (kRes: add vs for the residual block): xk (gathered
+// once), xkm1, xkp1, and the FP64 y in and out when it is flushed; the
+// first launch reads xk and writes xkp1 and y
+inline double cheb_vec_bytes(int yflush, bool first, int vs) {
+  return first ? 2.0 * vs + 8.0 : 3.0 * vs + (yflush ? 16.0 : 0.0);
+}
 
 // dst = src for bytes / 16 16-byte words (the bandwidth reference kernel)
 hipError_t stream_copy(size_t bytes, const void* src, void* dst, hipStream_t st);

@@ -91,8 +91,12 @@ struct Context {
   int filter_prec = EVSL_PREC_FP64;
   int refine_passes = 1;
   double refine_guard = 0.6;
+  // the first refinement pass filters with the polynomial of the slice's
+  // interval widened by refine_widen slice widths on each side (lower
+  // degree); 0 uses the slice's own filter
+  double refine_widen = 2.0;
   int mixed_min_deg = 32;    // mixed form: filters of degree outside
-  int mixed_max_deg = 3000;  // [mixed_min_deg, mixed_max_deg] run in FP64
+  int mixed_max_deg = 20000; // [mixed_min_deg, mixed_max_deg] run in FP64
   bool spmm_tuning = true;   // filter launches go through the matrix's layout tuner
   // scratch for reductions
   DevBuf red_part, red_scalars;

@@ -300,17 +300,22 @@ int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, doub
   const int n = c.n;
   const size_t vs = f32 ? 4 : 8;
   const size_t nb = (size_t)n * B;
+  // ring of three blocks and the residual block of the vector size, y FP64
   DevBuf X, coef;
-  EVSL_HIP(X.alloc(5 * nb * vs));
+  const size_t xbytes = 4 * nb * vs + nb * sizeof(double);
+  EVSL_HIP(X.alloc(xbytes));
   EVSL_HIP(coef.alloc(256 * sizeof(double)));
   char* base = X.as<char>();
-  EVSL_HIP(hipMemsetAsync(base, 0, 5 * nb * vs, c.stream));
+  EVSL_HIP(hipMemsetAsync(base, 0, xbytes, c.stream));
   EVSL_HIP(evsl_dev::fill(256, 0.01, coef.as<double>(), c.stream));
   // coefficient rows with room for the look-back of a 3-term y flush
   const double* scol = coef.as<double>() + 64;
   const double* mu0col = coef.as<double>() + 160;
+  // buf[3] = y (FP64), buf[4] = v0 / R
   void* buf[5];
-  for (int i = 0; i < 5; ++i) buf[i] = base + i * nb * vs;
+  for (int i = 0; i < 3; ++i) buf[i] = base + i * nb * vs;
+  buf[4] = base + 3 * nb * vs;
+  buf[3] = base + 4 * nb * vs;
   const evsl_dev::Mode md =
       mode == 2 ? evsl_dev::kKpm : (mode == 3 ? evsl_dev::kRes : evsl_dev::kCheb);
   // steady state of a long series: y is flushed (3 terms) every third launch
@@ -345,15 +350,14 @@ int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, doub
   (void)hipEventDestroy(e1);
   *ms_per_step = ms / nsteps;
   // per launch: KPM xk (gathered once), xkm1, v0 in, xkp1 out; ChebAv xk,
-  // xkm1 in, xkp1 out, plus y in and out on a flush; the residual
+  // xkm1 in, xkp1 out, plus the FP64 y in and out on a flush; the residual
   // recurrence adds the residual block R
-  double streams;
+  double vbytes;
   if (mode == 2)
-    streams = 4.0;
+    vbytes = 4.0 * vs;
   else
-    streams = 3.0 + 2.0 * flushes / nsteps + (mode == 3 ? 1.0 : 0.0);
-  *bytes_per_step =
-      (double)c.A.nnz * (vs + c.A.idx_bytes()) + 4.0 * (n + 1) + streams * nb * vs;
+    vbytes = 3.0 * vs + 16.0 * flushes / nsteps + (mode == 3 ? vs : 0.0);
+  *bytes_per_step = (double)c.A.nnz * (vs + c.A.idx_bytes()) + 4.0 * (n + 1) + vbytes * nb;
   return 0;
 }
 

@@ -76,9 +76,10 @@ struct EventClock {
   }
 };
 
-double filter_step_bytes(const Context& c, int B, int sv, int sa, int streams) {
-  return (double)c.A.nnz * (sa + c.A.idx_bytes()) + 4.0 * (c.n + 1) +
-         (double)streams * B * c.n * sv;
+// matrix bytes of one launch (values of sa bytes) plus vec_bytes per row
+// and column of the B-column block
+double filter_step_bytes(const Context& c, int B, int sa, double vec_bytes) {
+  return (double)c.A.nnz * (sa + c.A.idx_bytes()) + 4.0 * (c.n + 1) + vec_bytes * B * c.n;
 }
 
 }  // namespace
@@ -110,8 +111,10 @@ int cheb_av(const polparams* pol, const double* v, double* y, ChebWork& w, int p
     const size_t vs = f32 ? sizeof(float) : sizeof(double);
     FilterOp op;
     if (int e = make_filter_op(cc, pol->dd, f32, &op)) return e;
+    // the kernel accumulates y in FP64 in both precisions, straight into
+    // the caller's vector
     DevBuf& store = f32 ? w.b32 : w.bufs;
-    const int nvec = f32 ? 5 : 3;
+    const int nvec = f32 ? 4 : 3;
     EVSL_HIP(store.alloc((size_t)nvec * n * vs));
     char* base = store.as<char>();
     void* buf[3] = {base, base + (size_t)n * vs, base + 2 * (size_t)n * vs};
@@ -119,7 +122,6 @@ int cheb_av(const polparams* pol, const double* v, double* y, ChebWork& w, int p
     void* yout = y;
     if (f32) {
       void* v32 = base + 3 * (size_t)n * vs;
-      yout = base + 4 * (size_t)n * vs;
       EVSL_HIP(evsl_dev::f64_to_f32(n, v, (float*)v32, c.stream));
       vin = v32;
     }
@@ -132,9 +134,8 @@ int cheb_av(const polparams* pol, const double* v, double* y, ChebWork& w, int p
                                    mu_d + k, mu_d, k == 1 ? op.t1 : op.t2, op.cc, k == 1, yf,
                                    nullptr, c.stream));
       c.astats.filter_bytes +=
-          filter_step_bytes(c, 1, (int)vs, (int)vs, evsl_dev::cheb_streams(yf, k == 1));
+          filter_step_bytes(c, 1, (int)vs, evsl_dev::cheb_vec_bytes(yf, k == 1, (int)vs));
     }
-    if (f32) EVSL_HIP(evsl_dev::f32_to_f64(n, (const float*)yout, y, c.stream));
     c.astats.n_filter_steps += m;
     c.astats.n_filter_cols += m;
     return 0;

@@ -550,6 +550,11 @@ int evsl_amd_set_refine_guard(double guard) {
   ctx().refine_guard = guard;
   return 0;
 }
+int evsl_amd_set_refine_widen(double widen) {
+  if (!(widen >= 0.0 && widen <= 1e6)) return EVSL_ERR_ARG;
+  ctx().refine_widen = widen;
+  return 0;
+}
 int evsl_amd_set_mixed_degree_range(int min_deg, int max_deg) {
   if (min_deg < 0 || max_deg < min_deg) return EVSL_ERR_ARG;
   ctx().mixed_min_deg = min_deg;

@@ -60,7 +60,7 @@ __device__ __forceinline__ void cheb_spmm_block(
     const int blk, const int* __restrict__ ia, const int* __restrict__ ja,
     const short* __restrict__ jd, const VT* __restrict__ av, const int2* __restrict__ rbp,
     const VT* __restrict__ xk, const VT* __restrict__ xkm1, VT* __restrict__ xkp1,
-    VT* __restrict__ y, const VT* __restrict__ v0, const double* __restrict__ scol,
+    double* __restrict__ y, const VT* __restrict__ v0, const double* __restrict__ scol,
     const double* __restrict__ mu0col, double t_, double cc_, int first, int yflush,
     double* __restrict__ part, int* s_ia = nullptr) {
   static_assert(B % V == 0, "V must divide B");
@@ -68,6 +68,7 @@ __device__ __forceinline__ void cheb_spmm_block(
   constexpr int NG = kWG / L;  // lanes >= NG*L idle when L does not divide 256
   constexpr int U = (V * sizeof(VT) >= 32) ? 4 : 8;  // gathers in flight per lane
   using Vec = VecT<VT, V>;
+  using VecY = VecT<double, V>;
   // staging area for ja/a, reused for the group reductions once the row
   // walk is over
   // column indices: int32, or (I16) 16-bit offsets from the row, ja = row + jd
@@ -99,24 +100,29 @@ __device__ __forceinline__ void cheb_spmm_block(
   // y accumulates the last yflush terms T_{k-1}, T_k, T_{k+1} of the series
   // (0: y untouched); s, s1, s2 are their coefficients.  kCheb keeps the
   // per-degree coefficient rows B apart (scol - B, scol - 2B), kRes the
-  // (mu_k, mu_1) pairs 2 apart (mu0col - 2, mu0col - 4).
-  VT s[V], s1[V], s2[V], mu0[V], g[V];
+  // (mu_k, mu_1) pairs 2 apart (mu0col - 2, mu0col - 4).  y and its
+  // coefficients are FP64 whatever the vector type: with FP32 vectors the
+  // sum of a long series would otherwise carry the rounding of every one
+  // of its additions, noise that the filter does not damp (each term is
+  // widened to FP64 and added there).
+  double s[V], s1[V], s2[V], mu0[V];
+  VT g[V];
 #pragma unroll
   for (int j = 0; j < V; ++j) {
     s[j] = s1[j] = s2[j] = 0;
     mu0[j] = 0;
     g[j] = 0;
     if (MODE == kCheb) {
-      s[j] = (VT)scol[c0 + j];
-      if (yflush >= 2) s1[j] = (VT)scol[c0 + j - B];
-      if (yflush >= 3) s2[j] = (VT)scol[c0 + j - 2 * B];
-      if (first) mu0[j] = (VT)mu0col[c0 + j];
+      s[j] = scol[c0 + j];
+      if (yflush >= 2) s1[j] = scol[c0 + j - B];
+      if (yflush >= 3) s2[j] = scol[c0 + j - 2 * B];
+      if (first) mu0[j] = mu0col[c0 + j];
     } else if (MODE == kRes) {
       g[j] = (VT)scol[c0 + j];
-      s[j] = (VT)mu0col[0];
-      if (yflush >= 2) s1[j] = (VT)mu0col[-2];
-      if (yflush >= 3) s2[j] = (VT)mu0col[-4];
-      if (first) mu0[j] = (VT)mu0col[1];
+      s[j] = mu0col[0];
+      if (yflush >= 2) s1[j] = mu0col[-2];
+      if (yflush >= 3) s2[j] = mu0col[-4];
+      if (first) mu0[j] = mu0col[1];
     }
   }
   const bool ymode = MODE == kCheb || MODE == kRes;
@@ -127,28 +133,35 @@ __device__ __forceinline__ void cheb_spmm_block(
 
   // epilogue operands of a row, loaded ahead of the gathers
   struct Ops {
-    Vec xr, km1, yo, rr, v;
+    Vec xr, km1, rr, v;
+    VecY yo;
   };
-  using EV = typename ExtVec<VT, V>::type;
   auto ld = [](const VT* p, size_t i) { return *reinterpret_cast<const Vec*>(p + i); };
-  // the same load / store, non-temporal when NT
-  auto ld_last = [](const VT* p, size_t i) {
+  // the same load / store, non-temporal when NT (vectors of VT or of double)
+  auto ld_last = [](const auto* p, size_t i) {
+    using ET = typename std::remove_cv<typename std::remove_reference<decltype(*p)>::type>::type;
+    using RV = VecT<ET, V>;
     if constexpr (NT) {
+      using EV = typename ExtVec<ET, V>::type;
       const EV e = __builtin_nontemporal_load(reinterpret_cast<const EV*>(p + i));
-      Vec r;
-      __builtin_memcpy(&r, &e, sizeof(Vec));
+      RV r;
+      __builtin_memcpy(&r, &e, sizeof(RV));
       return r;
     } else {
-      return *reinterpret_cast<const Vec*>(p + i);
+      return *reinterpret_cast<const RV*>(p + i);
     }
   };
-  auto st_last = [](VT* p, size_t i, const Vec& v) {
+  auto st_last = [](auto* p, size_t i, const auto& v) {
+    using ET = typename std::remove_reference<decltype(*p)>::type;
+    using RV = VecT<ET, V>;
+    static_assert(sizeof(v) == sizeof(RV), "store width");
     if constexpr (NT) {
+      using EV = typename ExtVec<ET, V>::type;
       EV e;
-      __builtin_memcpy(&e, &v, sizeof(Vec));
+      __builtin_memcpy(&e, &v, sizeof(RV));
       __builtin_nontemporal_store(e, reinterpret_cast<EV*>(p + i));
     } else {
-      *reinterpret_cast<Vec*>(p + i) = v;
+      *reinterpret_cast<RV*>(p + i) = v;
     }
   };
   auto prefetch = [&](int r) {
@@ -179,13 +192,13 @@ __device__ __forceinline__ void cheb_spmm_block(
     if (ymode) {
       if (yflush > 0) {
         // the same additions, in the same order, as one update per degree
-        Vec yn;
+        VecY yn;
 #pragma unroll
         for (int j = 0; j < V; ++j) {
-          VT yv = first ? mu0[j] * o.xr.v[j] : o.yo.v[j];
-          if (yflush >= 3) yv = yv + s2[j] * o.km1.v[j];
-          if (yflush >= 2) yv = yv + s1[j] * o.xr.v[j];
-          yn.v[j] = yv + s[j] * w.v[j];
+          double yv = first ? mu0[j] * (double)o.xr.v[j] : o.yo.v[j];
+          if (yflush >= 3) yv = yv + s2[j] * (double)o.km1.v[j];
+          if (yflush >= 2) yv = yv + s1[j] * (double)o.xr.v[j];
+          yn.v[j] = yv + s[j] * (double)w.v[j];
         }
         st_last(y, i, yn);
       }
@@ -330,7 +343,7 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
     const int* __restrict__ ia, const int* __restrict__ ja, const short* __restrict__ jd,
     const VT* __restrict__ av,
     const int2* __restrict__ rbp, int nblk, const VT* __restrict__ xk,
-    const VT* __restrict__ xkm1, VT* __restrict__ xkp1, VT* __restrict__ y,
+    const VT* __restrict__ xkm1, VT* __restrict__ xkp1, double* __restrict__ y,
     const VT* __restrict__ v0, const double* __restrict__ scol,
     const double* __restrict__ mu0col, double t_, double cc_, int first, int yflush,
     double* __restrict__ part, int stage_ia) {
@@ -342,8 +355,8 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
 
 // Non-temporal streams (the NT instances) exist for the ChebAv and residual
 // steps of 8 or more columns.  They are chosen when one launch's working set
-// -- the matrix, and the vector blocks xk, xkm1, xkp1, y (and R) -- exceeds
-// 320 MB: beyond the 256 MB Infinity Cache the block no longer stays
+// -- the matrix, and the vector blocks xk, xkm1, xkp1 (and R) plus the FP64
+// y -- exceeds 320 MB: beyond the 256 MB Infinity Cache the block no longer stays
 // resident from launch to launch, and keeping the last-use streams out of L2
 // saves the x rows re-read across planes.  Measured on the 100^3 Laplacian
 // (µs per launch, plain -> non-temporal): FP32 32-column ChebAv 137.5 -> 124.8
@@ -356,8 +369,9 @@ constexpr bool kStreamingCapable = (MODE == kCheb || MODE == kRes) && B >= 8;
 
 inline bool streaming_launch(const Csr& A, int B, int vsize, int mode) {
   if (!((mode == kCheb || mode == kRes) && B >= 8)) return false;
-  const double streams = mode == kRes ? 5.0 : 4.0;
-  const double ws = streams * (double)A.n * B * vsize +
+  // xk, xkm1, xkp1 (and R) of the vector type, y in FP64
+  const double streams = mode == kRes ? 4.0 : 3.0;
+  const double ws = streams * (double)A.n * B * vsize + 8.0 * A.n * B +
                     (double)A.nnz * (vsize + (A.jd ? 2 : 4)) + 4.0 * (A.n + 1);
   return ws > 320e6;
 }
@@ -387,7 +401,7 @@ hipError_t launch_variant(const Csr& A, const void* xk, const void* xkm1, void* 
 #define EVSL_SPMM_LAUNCH(I16_, NT_)                                                             \
   hipLaunchKernelGGL((k_cheb_spmm<VT, B, V, MODE, LDS, I16_, NT_>), dim3(nblk), dim3(kWG), dyn, \
                      st, A.ia, A.ja, A.jd, av, rb, nblk, (const VT*)xk, (const VT*)xkm1,      \
-                     (VT*)xkp1, (VT*)y, (const VT*)v0, s, mu0, t, cc, f, yflush, part, stage_ia)
+                     (VT*)xkp1, (double*)y, (const VT*)v0, s, mu0, t, cc, f, yflush, part, stage_ia)
   if constexpr (kStreamingCapable<MODE, B>) {
     if (streaming_launch(A, B, (int)sizeof(VT), MODE)) {
       if (A.jd)

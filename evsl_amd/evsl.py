@@ -38,7 +38,8 @@ EXPORTS = [
     "randn_double_device", "sort_double", "linspace", "evsl_timer", "StatsPrint",
     "StatsReset",
     "evsl_amd_set_filter_precision", "evsl_amd_get_filter_precision",
-    "evsl_amd_set_refine_passes", "evsl_amd_set_refine_guard", "evsl_amd_get_stats", "evsl_amd_reset_stats",
+    "evsl_amd_set_refine_passes", "evsl_amd_set_refine_guard", "evsl_amd_set_refine_widen",
+    "evsl_amd_get_stats", "evsl_amd_reset_stats",
     "evsl_amd_last_error", "evsl_amd_stats_size", "kpmdos_ex", "evsl_amd_bench_filter",
     "ChebLanNr_batch", "evsl_amd_read_coo_mm", "evsl_amd_bench_copy",
     "evsl_amd_spmm_vec", "evsl_amd_spmm_index_bytes", "evsl_amd_cheb_av_cols",
@@ -57,7 +58,9 @@ class AmdStats(C.Structure):
                 ("n_filter_cols", C.c_size_t), ("n_lanczos", C.c_size_t),
                 ("n_reorth_pass", C.c_size_t), ("filter_bytes", C.c_double),
                 ("reorth_bytes", C.c_double), ("n_refine_cols", C.c_size_t),
-                ("n_refine_extra", C.c_size_t), ("refine_bytes", C.c_double)]
+                ("n_refine_extra", C.c_size_t), ("refine_bytes", C.c_double),
+                ("n_refine_degcols", C.c_size_t), ("n_fp64_slices", C.c_size_t),
+                ("n_fp64_reruns", C.c_size_t)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -120,6 +123,7 @@ class EVSL:
             "evsl_amd_get_filter_precision": (I, []),
             "evsl_amd_set_refine_passes": (I, [I]),
             "evsl_amd_set_refine_guard": (I, [D]),
+            "evsl_amd_set_refine_widen": (I, [D]),
             "evsl_amd_set_mixed_degree_range": (I, [I, I]),
             "evsl_amd_get_stats": (I, [C.POINTER(AmdStats)]),
             "evsl_amd_reset_stats": (V, []),
@@ -391,6 +395,9 @@ class EVSL:
 
     def set_refine_guard(self, g):
         return self.lib.evsl_amd_set_refine_guard(g)
+
+    def set_refine_widen(self, f):
+        return self.lib.evsl_amd_set_refine_widen(f)
 
     def stats(self):
         s = AmdStats()

@@ -34,6 +34,12 @@ typedef struct _evslAmdStats {
   size_t n_refine_cols;  /* Ritz-vector columns filtered by refinement     */
   size_t n_refine_extra; /* refinement passes beyond the requested ones   */
   double refine_bytes;   /* algorithmic bytes of the refinement's filter  */
+  size_t n_refine_degcols; /* sum over refinement passes of columns x degree */
+  size_t n_fp64_slices;  /* MIXED: slices filtered in FP64 from the start  */
+                         /* (degree outside the mixed range, or a capped   */
+                         /* filter)                                        */
+  size_t n_fp64_reruns;  /* MIXED: slices redone in FP64 after a stalled   */
+                         /* refinement                                     */
 } evslAmdStats;
 
 #ifdef __cplusplus
@@ -50,16 +56,22 @@ int evsl_amd_set_refine_passes(int passes);
  * guard * bar enter the FP64 Rayleigh-Ritz space (0 < guard <= 1,
  * default 0.6) */
 int evsl_amd_set_refine_guard(double guard);
+/* the first refinement pass filters with find_pol's polynomial for the
+ * slice's interval widened by `widen` slice widths on each side (clipped to
+ * [lmin, lmax]): a lower degree, enough to damp the FP32 filter's rounding
+ * noise far from the interval, which is what the residuals weigh most;
+ * further passes, while a Ritz value in [a, b] is near convergence, use the
+ * slice's own filter.  0 makes every pass use the slice's filter (default 2) */
+int evsl_amd_set_refine_widen(double widen);
 /* MIXED mode: only slices whose filter degree lies in [min_deg, max_deg]
- * filter in FP32 (default [32, 3000]); the others run in FP64 -- a
+ * filter in FP32 (default [32, 20000]); the others run in FP64 -- a
  * low-degree filter carries few of the bytes and separates the refinement's
- * guard set too weakly, and at very high degrees the refinement's FP32
- * residual recurrence accumulates too much rounding.  A filter that did not
- * meet find_pol's thresholds (deg = max_deg - 1) also runs in FP64: it
- * separates its interval weakly and its slice tends to end at maxit, where
- * the FP32-filtered basis leaves pairs just above tol.  A mixed slice whose
- * refinement stalls above tol after its extra passes is redone from its
- * start vector in FP64 (counted in evslAmdStats.n_refine_extra). */
+ * guard set too weakly.  A filter that did not meet find_pol's thresholds
+ * (deg = max_deg - 1) also runs in FP64: it separates its interval weakly
+ * and its slice tends to end at maxit, where the FP32-filtered basis leaves
+ * pairs just above tol (both counted in evslAmdStats.n_fp64_slices).  A
+ * mixed slice whose refinement stalls above tol after its extra passes is
+ * redone from its start vector in FP64 (evslAmdStats.n_fp64_reruns). */
 int evsl_amd_set_mixed_degree_range(int min_deg, int max_deg);
 
 int evsl_amd_get_stats(evslAmdStats *s);

@@ -118,15 +118,18 @@ def test_mixed_low_degree_end_interval_matches_fp64(ev):
     ev.set_precision(PREC_MIXED)
     try:
         for mindeg in (32, 1):
-            assert ev.lib.evsl_amd_set_mixed_degree_range(mindeg, 3000) == 0
+            assert ev.lib.evsl_amd_set_mixed_degree_range(mindeg, 20000) == 0
             ev.reset_stats()
             em, (rm,) = ev.ChebLanNr_batch([iv], 300, 1e-8, v0, [pol])
             assert em == 0, ev.last_error()
             assert np.array_equal(rm[0], r64[0]) and np.array_equal(rm[2], r64[2]), mindeg
+            st = ev.stats()
             if mindeg == 1:
-                assert ev.stats()["n_refine_extra"] >= 1   # the stalled slice was redone
+                assert st["n_fp64_reruns"] == 1 and st["n_fp64_slices"] == 0  # redone in FP64
+            else:
+                assert st["n_fp64_slices"] == 1 and st["n_fp64_reruns"] == 0  # FP64 from the start
     finally:
-        ev.lib.evsl_amd_set_mixed_degree_range(32, 3000)
+        ev.lib.evsl_amd_set_mixed_degree_range(32, 20000)
         ev.set_precision(PREC_FP64)
         ev.free_pol(pol)
 
@@ -209,5 +212,55 @@ def test_degenerate_spectrum_mixed_pairs_are_true_eigenpairs(ev):
         assert np.abs(Ym.T @ Ym - np.eye(len(lm))).max() < 1e-8
         R = S @ Ym - Ym * lm
         assert np.all(np.linalg.norm(R, axis=0) < 1e-8)
+    for p in pols:
+        ev.free_pol(p)
+
+
+def test_mixed_fp32_filter_at_high_degree_anderson(ev):
+    """The precision form where the filter degree is in the thousands: two
+    slices at the band centre of a 3-D Anderson model (18^3 periodic, W =
+    16.5) get filters of degree > 3000, which the mixed form runs in FP32
+    (no slice falls back to FP64).  Per-slice counts equal the dense
+    spectrum's and the FP64 path's, eigenvalues agree with the dense ones to
+    1e-10 and residuals stay below tol."""
+    A = M.anderson(18, 16.5, seed=77)
+    lo, hi = M.gershgorin(A)
+    assert ev.set_matrix(A) == 0
+    dense = np.linalg.eigvalsh(A.to_scipy().toarray())
+    edges = [-0.012, 0.0, 0.012]
+    ivs, pols = [], []
+    for a, b in zip(edges[:-1], edges[1:]):
+        iv = np.array([a, b, lo, hi])
+        pol = ev.default_pol()
+        pol.damping, pol.thresh_int, pol.thresh_ext, pol.max_deg = 2, 0.8, 0.2, 10000
+        assert ev.find_pol(iv, pol) == 0
+        assert pol.deg > 3000, pol.deg
+        ivs.append(iv)
+        pols.append(pol)
+    want = [int(np.sum((dense >= a) & (dense <= b))) for a, b in zip(edges[:-1], edges[1:])]
+    assert min(want) >= 3, want
+    v0 = np.random.default_rng(8).uniform(-1, 1, A.n)
+    tol = 1e-8
+    try:
+        ev.set_precision(PREC_FP64)
+        e64, r64 = ev.ChebLanNr_batch(ivs, 300, tol, v0, pols)
+        assert e64 == 0, ev.last_error()
+        ev.set_precision(PREC_MIXED)
+        ev.reset_stats()
+        em, rm = ev.ChebLanNr_batch(ivs, 300, tol, v0, pols)
+        assert em == 0, ev.last_error()
+        st = ev.stats()
+    finally:
+        ev.set_precision(PREC_FP64)
+    print(f"  degrees {[p.deg for p in pols]}, counts dense {want}, "
+          f"fp64 {[len(r[0]) for r in r64]}, mixed {[len(r[0]) for r in rm]}; refinement "
+          f"{st['n_refine_degcols']} column-degrees, {st['n_refine_extra']} extra passes",
+          flush=True)
+    assert st["n_fp64_slices"] == 0 and st["n_fp64_reruns"] == 0
+    for (a, b), w, (l64, _, _), (lm, _, res) in zip(zip(edges[:-1], edges[1:]), want, r64, rm):
+        ref = np.sort(dense[(dense >= a) & (dense <= b)])
+        assert len(l64) == w and len(lm) == w, (a, b, w, len(l64), len(lm))
+        assert np.max(np.abs(np.sort(lm) - ref)) <= 1e-10
+        assert np.max(res) < tol
     for p in pols:
         ev.free_pol(p)

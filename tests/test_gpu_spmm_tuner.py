@@ -116,7 +116,7 @@ def test_every_layout_same_bits_residual_step(ev):
     ev.set_precision(PREC_MIXED)
     # both slices on the FP32 filter (and so through the refinement) whatever
     # their degree
-    assert ev.lib.evsl_amd_set_mixed_degree_range(2, 3000) == 0
+    assert ev.lib.evsl_amd_set_mixed_degree_range(2, 20000) == 0
     try:
         ev.set_spmm_tuning(False)
         err, ref = ev.ChebLanNr_batch(ivs, 900, 1e-8, v0, pols)
@@ -134,7 +134,7 @@ def test_every_layout_same_bits_residual_step(ev):
                 assert np.array_equal(Y0, Y1), i
     finally:
         ev.set_precision(PREC_FP64)
-        ev.lib.evsl_amd_set_mixed_degree_range(32, 3000)
+        ev.lib.evsl_amd_set_mixed_degree_range(32, 20000)
         ev.set_spmm_tuning(True)
         reset_keys(ev)
     for p in pols:
