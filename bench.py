@@ -120,59 +120,144 @@ def run_step(ev, A, lo, hi, args, form):
             "dd": 0.5 * (hi - lo)}
 
 
-def cpu_baseline(A, lo, hi, step, args):
-    """Time the CPU oracle on a bounded sample of the step and extrapolate:
-    (a) 40 degrees of the KPM recurrence for one vector, (b) the first four
-    filtered Lanczos steps of slice 0 (five ChebAv applications at its
-    degree, start-vector filter included), (c) one CGS/DGKS pass against 128
-    basis vectors.  The full step is then priced from the Lanczos steps and
-    degrees the GPU run took."""
+def cpu_worker(spec):
+    """One CPU-baseline sample in its own process (the oracle keeps global
+    state, so slices run in parallel as processes, one thread each): filtered
+    Lanczos steps of one slice, optionally KPM degrees and a CGS pass.
+    Prints one JSON line."""
     from oracle.pyoracle import Oracle
+    A, lo, hi = build_problem(spec["nx"], seed=2024)
     orc = Oracle()
     orc.set_csr(A)
-    n = A.n
-    intv = np.array([args.a, args.b, lo, hi])
-    kdeg = 40
-    t0 = orc.timer()
-    orc.kpmdos(kdeg, 1, 1, intv)
-    t_kpm_deg = (orc.timer() - t0) / kdeg
-    s0 = step["slices"][0]
-    iv = np.array([s0["a"], s0["b"], lo, hi])
+    out = {"slice": spec["slice"]}
+    if spec.get("kdeg"):
+        intv = np.array([spec["a"], spec["b"], lo, hi])
+        t0 = orc.timer()
+        orc.kpmdos(spec["kdeg"], 1, 1, intv)
+        out["t_kpm_deg"] = (orc.timer() - t0) / spec["kdeg"]
+    iv = np.array([spec["sa"], spec["sb"], lo, hi])
     pol = orc.default_pol()
     pol.damping, pol.thresh_int, pol.thresh_ext, pol.max_deg = 2, 0.8, 0.2, 3000
     orc.find_pol(iv, pol)
-    v0 = np.random.default_rng(11).uniform(-1.0, 1.0, n)
-    lan_steps = 4
+    v0 = np.random.default_rng(11).uniform(-1.0, 1.0, A.n)
+    steps = spec["lan_steps"]
     t0 = orc.timer()
-    orc.ChebLanNr(iv, lan_steps, args.tol, v0, pol)
+    orc.ChebLanNr(iv, steps, spec["tol"], v0, pol)
     t_lan = orc.timer() - t0
+    out["deg"] = int(pol.deg)
+    out["t_lan"] = t_lan
+    out["t_deg"] = t_lan / ((steps + 1.0) * pol.deg)  # ChebAv calls at this degree
     orc.free_pol(pol)
-    t_deg = t_lan / ((lan_steps + 1.0) * s0["deg"])   # ChebAv calls at this degree
-    k = 128
-    Q = np.random.default_rng(0).standard_normal((n, k))
-    w = np.random.default_rng(1).standard_normal(n)
-    t0 = orc.timer()
-    orc.CGS_DGKS(Q, w, i_max=1)
-    t_col = (orc.timer() - t0) / k
-    del Q
-    est = args.mdeg * args.nvec * t_kpm_deg
-    avg_steps = step["stats"]["n_lanczos"] / max(1, len(step["slices"]))
-    # CGS passes per Lanczos step as the GPU run measured them (DGKS adds a
-    # second pass only when the first leaves too little of the vector)
+    if spec.get("cgs_k"):
+        k = spec["cgs_k"]
+        Q = np.random.default_rng(0).standard_normal((A.n, k))
+        w = np.random.default_rng(1).standard_normal(A.n)
+        t0 = orc.timer()
+        orc.CGS_DGKS(Q, w, i_max=1)
+        out["t_col"] = (orc.timer() - t0) / k
+    print(json.dumps(out), flush=True)
+
+
+def cpu_info():
+    """Host CPU as the baseline ran on it: model, logical CPUs, and the
+    CPUs this process may use."""
+    import platform
+    import subprocess
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "machine": platform.machine()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket",
+                             "Thread(s) per core", "NUMA node(s)"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(A, lo, hi, step, args):
+    """Time the CPU oracle (C, -O2) on bounded samples of the step and price
+    the full step from the Lanczos steps and degrees the GPU run took.
+
+    1 thread: alone, 40 KPM degrees of one vector, args.cpu_lan_steps
+    filtered Lanczos steps of slice 0 (ChebAv at its degree, start-vector
+    filter included) and one CGS pass against 128 basis vectors.
+    All cores: the reference's threading model (TESTS/PLanR/MMPLanR_omp.c:184,
+    one slice per thread, threads = min(slices, cores); KPM vectors split
+    over the threads) -- the slices' samples run at once, one process each,
+    half as many Lanczos steps, so the per-degree time includes the memory
+    contention of all cores; each slice is priced at its own measured
+    per-degree time and the step is its slowest slice plus the KPM phase."""
+    import subprocess
+    n = A.n
+    slices = step["slices"]
+    base = {"nx": args.nx, "a": args.a, "b": args.b, "tol": args.tol}
+
+    def spec(s, steps, kdeg=0, cgs_k=0):
+        d = dict(base, slice=s, sa=slices[s]["a"], sb=slices[s]["b"], lan_steps=steps,
+                 kdeg=kdeg, cgs_k=cgs_k)
+        return [sys.executable, os.path.abspath(__file__), "--cpu-worker", json.dumps(d)]
+
+    def run(cmds):
+        procs = [subprocess.Popen(c, stdout=subprocess.PIPE, text=True) for c in cmds]
+        outs = []
+        for p in procs:
+            o, _ = p.communicate()
+            if p.returncode != 0:
+                raise RuntimeError(f"cpu baseline worker failed ({p.returncode})")
+            outs.append(json.loads(o.strip().splitlines()[-1]))
+        return outs
+
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    P = max(1, min(len(slices), cores))
+    t0 = time.perf_counter()
+    alone = run([spec(0, args.cpu_lan_steps, kdeg=40, cgs_k=128)])[0]
+    t_alone = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    par = run([spec(s, max(2, args.cpu_lan_steps // 2), kdeg=20 if s == 0 else 0)
+               for s in range(P)])
+    t_par = time.perf_counter() - t0
+    avg_steps = step["stats"]["n_lanczos"] / max(1, len(slices))
+    # CGS passes per Lanczos step as the GPU run measured them
     passes = step["stats"]["n_reorth_pass"] / max(1, step["stats"]["n_lanczos"])
-    for s in step["slices"]:
+    t_col = alone["t_col"]
+    pairs = sum(s["nev"] for s in slices)
+
+    def slice_secs(s, t_deg, cgs_scale):
         steps = s.get("steps", avg_steps)
-        est += (steps + 1) * s["deg"] * t_deg
-        est += passes * t_col * steps * (steps + 1) / 2.0
-    sample_secs = t_kpm_deg * kdeg + t_lan + t_col * k
-    total_pairs = sum(s["nev"] for s in step["slices"])
-    return {"value": total_pairs / est, "unit": "eigenpairs/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle (C, -O2, 1 thread) on the same matrix: {kdeg} KPM degrees, "
-                       f"{lan_steps} filtered Lanczos steps of slice 0 (deg {s0['deg']}), 1 CGS "
-                       f"pass vs {k} vectors ({sample_secs:.1f}s); full step extrapolated to "
-                       f"{est:.0f}s from the GPU run's per-slice Lanczos steps, degrees and "
-                       f"CGS passes per step ({passes:.2f})"),
-            "extrapolated_step_s": est}
+        return (steps + 1) * s["deg"] * t_deg + cgs_scale * passes * t_col * steps * (steps + 1) / 2
+
+    # 1 thread: everything in sequence at the alone rates
+    est1 = args.mdeg * args.nvec * alone["t_kpm_deg"]
+    for s in slices:
+        est1 += slice_secs(s, alone["t_deg"], 1.0)
+    # all cores: slices on their own cores at the contended per-degree rate
+    # (slices beyond the P sampled ones use the sampled mean), the CGS
+    # passes slowed by the same factor; KPM vectors split over P threads
+    t_par_deg = [w["t_deg"] for w in par]
+    slow = float(np.mean(t_par_deg)) / alone["t_deg"]
+    est_all_slices = max(slice_secs(s, t_par_deg[i] if i < P else float(np.mean(t_par_deg)), slow)
+                         for i, s in enumerate(slices))
+    est_all = args.mdeg * -(-args.nvec // P) * par[0]["t_kpm_deg"] + est_all_slices
+    return {"value": pairs / est1, "unit": "eigenpairs/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle (C, -O2, 1 thread) on the same matrix, alone: 40 KPM degrees, "
+                       f"{args.cpu_lan_steps} filtered Lanczos steps of slice 0 (deg "
+                       f"{alone['deg']}, {alone['t_lan']:.1f}s), 1 CGS pass vs 128 vectors "
+                       f"({t_alone:.1f}s wall incl. matrix set-up); full step extrapolated to "
+                       f"{est1:.0f}s from the GPU run's per-slice Lanczos steps, degrees and CGS "
+                       f"passes per step ({passes:.2f})"),
+            "extrapolated_step_s": est1,
+            "all_cores": {"value": pairs / est_all, "unit": "eigenpairs/s", "cores": P,
+                          "sample": (f"{P} slices at once, one process each "
+                                     f"(TESTS/PLanR/MMPLanR_omp.c:184 threading), "
+                                     f"{max(2, args.cpu_lan_steps // 2)} Lanczos steps each "
+                                     f"({t_par:.1f}s wall); per-degree time under contention "
+                                     f"{slow:.2f}x the 1-thread rate; step priced as its slowest "
+                                     f"slice plus KPM over {P} threads"),
+                          "extrapolated_step_s": est_all},
+            "cpu": cpu_info()}
 
 
 def kpm_batch_bench(args):
@@ -580,6 +665,12 @@ def main():
                    help="timed form (the FP64 plain form is also run once for the ratio)")
     p.add_argument("--no-plain", action="store_true", help="skip the FP64 plain reference run")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-lan-steps", type=int, default=20,
+                   help="filtered Lanczos steps of the CPU baseline's 1-thread sample (the "
+                        "all-core sample runs half as many per slice)")
+    p.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
+    p.add_argument("--no-batch64", action="store_true",
+                   help="skip the FP64 batched reference run")
     p.add_argument("--maxit", type=int, default=0,
                    help="cap the Krylov dimension per slice (short profiling runs only)")
     p.add_argument("--filter-only", action="store_true",
@@ -611,6 +702,9 @@ def main():
                    help="run the fused filter kernel on its fixed layout policy (no per-matrix "
                         "layout tuning)")
     args = p.parse_args()
+    if args.cpu_worker is not None:  # CPU-baseline sample (no device use)
+        cpu_worker(json.loads(args.cpu_worker))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
                                    "only (slices shard with no communication)",
@@ -683,6 +777,15 @@ def main():
         plain = run_step(ev, A, lo, hi, args, "plain")
         plain["wall"] = time.perf_counter() - tp
         log(f"[bench] plain: {sum(s['nev'] for s in plain['slices'])} pairs in {plain['wall']:.1f}s")
+    b64 = None
+    if args.form == "mixed" and not args.no_batch64:
+        # the same batched solver in FP64: what batching alone gives, so the
+        # FP32 filter's share of the speedup is visible
+        log("[bench] FP64 batched form, once")
+        tp = time.perf_counter()
+        b64 = run_step(ev, A, lo, hi, args, "batch64")
+        b64["wall"] = time.perf_counter() - tp
+        log(f"[bench] batch64: {sum(s['nev'] for s in b64['slices'])} pairs in {b64['wall']:.1f}s")
 
     # dominant kernel of the timed form: the fused Chebyshev-step SpMM
     if args.form == "mixed":
@@ -725,8 +828,12 @@ def main():
                         "ritz_refine": st["t_ritz"]} if args.form != "plain" else None,
         "counts": {"reorth_passes": int(st["n_reorth_pass"]),
                    "refine_columns": int(st["n_refine_cols"]),
+                   "refine_column_degrees": int(st["n_refine_degcols"]),
                    "refine_extra_passes": int(st["n_refine_extra"]),
-                   "refine_GB": st["refine_bytes"] / 1e9} if args.form != "plain" else None,
+                   "refine_GB": st["refine_bytes"] / 1e9,
+                   "slices_fp64_under_mixed": int(st["n_fp64_slices"]),
+                   "slices_rerun_in_fp64": int(st["n_fp64_reruns"])}
+        if args.form != "plain" else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kname, "layout_VR": klayout, "launch_ms": ms,
@@ -757,6 +864,18 @@ def main():
                 "pairs": int(d.size), "p50": float(np.percentile(d, 50)),
                 "p90": float(np.percentile(d, 90)), "p99": float(np.percentile(d, 99)),
                 "max": float(d.max()), "per_slice_max": diffs}
+    if b64 is not None:
+        bc = [s["nev"] for s in b64["slices"]]
+        bv = sum(bc) / b64["wall"]
+        bst = b64["stats"]
+        rec["fp64_batched"] = {"value": bv, "wall_s": b64["wall"], "solve_s": b64["t_solve"],
+                               "per_slice_counts": bc,
+                               "max_residual": max(s["max_res"] for s in b64["slices"]),
+                               "breakdown_s": {"filter": bst["t_filter"],
+                                               "reorth": bst["t_reorth"],
+                                               "ritz": bst["t_ritz"]}}
+        rec["speedup_vs_fp64_batched"] = value / bv if bv > 0 else None
+        rec["counts_match_fp64_batched"] = bc == rec["per_slice_counts"]
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline sample")
         ref = plain if plain is not None else last

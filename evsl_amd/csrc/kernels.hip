@@ -769,10 +769,29 @@ hipError_t launch_cheb_B(const Csr& A, int B, const void* xk, const void* xkm1, 
 
 // STREAM-style copy with 16-byte accesses: the measured HBM reference the
 // filter kernel is compared with
-__global__ __launch_bounds__(kWG) void k_stream_copy(size_t n16, const uint4* __restrict__ src,
-                                                     uint4* __restrict__ dst) {
-  for (size_t i = (size_t)blockIdx.x * kWG + threadIdx.x; i < n16; i += (size_t)gridDim.x * kWG)
-    dst[i] = src[i];
+// The bandwidth yardstick: each thread keeps kCopyUnroll 16-byte loads in
+// flight (coalesced: word j of a thread sits kWG words after word j-1), and
+// loads and stores are non-temporal, so the copy streams through the caches
+// the way a well-formed bulk kernel does.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kCopyUnroll = 4;
+__global__ __launch_bounds__(kWG) void k_stream_copy(size_t n16, const u32x4* __restrict__ src,
+                                                     u32x4* __restrict__ dst) {
+  const size_t step = (size_t)gridDim.x * kWG * kCopyUnroll;
+  for (size_t base = (size_t)blockIdx.x * kWG * kCopyUnroll + threadIdx.x; base < n16;
+       base += step) {
+    u32x4 v[kCopyUnroll];
+#pragma unroll
+    for (int j = 0; j < kCopyUnroll; ++j) {
+      const size_t i = base + (size_t)j * kWG;
+      if (i < n16) v[j] = __builtin_nontemporal_load(src + i);
+    }
+#pragma unroll
+    for (int j = 0; j < kCopyUnroll; ++j) {
+      const size_t i = base + (size_t)j * kWG;
+      if (i < n16) __builtin_nontemporal_store(v[j], dst + i);
+    }
+  }
 }
 
 }  // namespace
@@ -783,9 +802,9 @@ __global__ __launch_bounds__(kWG) void k_stream_copy(size_t n16, const uint4* __
 hipError_t stream_copy(size_t bytes, const void* src, void* dst, hipStream_t st) {
   const size_t n16 = bytes / 16;
   if (n16 == 0) return hipSuccess;
-  const unsigned grid = grid_for(n16, kWG, 65536);
-  hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(kWG), 0, st, n16, (const uint4*)src,
-                     (uint4*)dst);
+  const unsigned grid = grid_for((n16 + kCopyUnroll - 1) / kCopyUnroll, kWG, 65536);
+  hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(kWG), 0, st, n16, (const u32x4*)src,
+                     (u32x4*)dst);
   return hipGetLastError();
 }
 
