@@ -380,6 +380,7 @@ def kb_slice_phase(args, ev, pool, owner, mus):
                 bh = mid
         sl.append((m, a, bh, min(A.n, max(5 * (target + 2), 300)), int(srng.integers(1 << 30))))
     out, counts = {}, {}
+    ev.lib.evsl_amd_set_grouped_filter(0 if args.no_group else 1)
     for name, prec in (("mixed", PREC_MIXED), ("fp64", PREC_FP64)):
         srv = Server(ev, 200 << 30, prec)
         t0 = time.perf_counter()
@@ -427,6 +428,7 @@ def kb_slice_phase(args, ev, pool, owner, mus):
              "mixed": counts["mixed"][i], "fp64": counts["fp64"][i]}
             for i in range(len(sl)) if counts["mixed"][i] != counts["fp64"][i]]
     return {"slices": len(sl), "eigs_per_slice": list(args.kb_slice_eigs), "tol": 1e-8,
+            "grouped_filter": not args.no_group,
             "mixed": out["mixed"], "fp64": out["fp64"],
             "speedup_eigenpairs_per_s": out["mixed"]["eigenpairs_per_s"] /
                                         out["fp64"]["eigenpairs_per_s"],
@@ -483,7 +485,7 @@ def serve_bench(args):
         m = int(rng.integers(len(pool)))
         A, lo, hi, mu = pool[m]
         if rng.uniform() < 0.7:
-            target = int(rng.integers(20, 101))
+            target = int(rng.integers(args.sv_eigs[0], args.sv_eigs[1] + 1))
             a = lo + (hi - lo) * rng.uniform(0.02, 0.8)
             bl, bh = a, hi
             for _ in range(60):  # the b with ~target eigenvalues in [a, b]
@@ -502,7 +504,10 @@ def serve_bench(args):
     arrivals = np.cumsum(rng.exponential(1.0 / args.sv_rate, len(reqs)))
     out = {}
     counts = {}
-    for name, prec in (("mixed", PREC_MIXED), ("fp64", PREC_FP64)):
+    ev.lib.evsl_amd_set_grouped_filter(0 if args.no_group else 1)
+    forms = [(nm, pr) for nm, pr in (("mixed", PREC_MIXED), ("fp64", PREC_FP64))
+             if nm in args.sv_forms]
+    for name, prec in forms:
         srv = Server(ev, 64 << 30, prec)
         mids = [srv.add_matrix(A, lo, hi) for A, lo, hi, _ in pool]
         assert min(mids) >= 0, ev.last_error()
@@ -555,25 +560,29 @@ def serve_bench(args):
                      "eigenpairs_per_s": pairs / span, "requests_per_s": len(reqs) / span}
         counts[name] = cnt
         log(f"[bench] serve {name}: {out[name]}")
+    both = "mixed" in out and "fp64" in out
     print(json.dumps({
-        "workload": f"request loop: {len(reqs)} requests (70% slices of 20-100 eigenvalues, tol "
+        "workload": f"request loop: {len(reqs)} requests (70% slices of {args.sv_eigs[0]}-"
+                    f"{args.sv_eigs[1]} eigenvalues, tol "
                     f"1e-6/1e-8; 30% DOS Mdeg 100-400) arriving as a Poisson stream of "
                     f"{args.sv_rate:g}/s over {len(pool)} resident matrices (2-D/3-D "
                     f"Laplacians, random symmetric), n in [{args.sv_n[0]:g}, {args.sv_n[1]:g}]",
-        "spmm_tuning": not args.no_tune,
-        "mixed": out["mixed"], "fp64": out["fp64"],
-        "speedup_eigenpairs_per_s": out["mixed"]["eigenpairs_per_s"] /
-                                    out["fp64"]["eigenpairs_per_s"],
-        "slice_counts_equal_mixed_vs_fp64": counts["mixed"] == counts["fp64"],
+        "spmm_tuning": not args.no_tune, "grouped_filter": not args.no_group,
+        "per_slice_counts": counts,
+        **{k: out[k] for k in out},
+        "speedup_eigenpairs_per_s": (out["mixed"]["eigenpairs_per_s"] /
+                                     out["fp64"]["eigenpairs_per_s"]) if both else None,
+        "slice_counts_equal_mixed_vs_fp64": counts["mixed"] == counts["fp64"] if both else None,
         "slice_count_differences": int(sum(a != b for a, b in zip(counts["mixed"],
-                                                                   counts["fp64"]))),
+                                                                   counts["fp64"])))
+        if both else None,
         "differing_slices": [
             {"matrix": r[1], "n": int(pool[r[1]][0].n),
              "family": ["2-D Laplacian", "3-D Laplacian", "random symmetric"][r[1] % 3],
              "interval": [float(r[2]), float(r[3])], "maxit": int(r[4]), "tol": r[5],
              "mixed": cm, "fp64": cf}
-            for r, cm, cf in zip([q for q in reqs if q[0] == "slice"], counts["mixed"],
-                                 counts["fp64"]) if cm != cf],
+            for r, cm, cf in zip([q for q in reqs if q[0] == "slice"],
+                                 counts.get("mixed", []), counts.get("fp64", [])) if cm != cf],
     }))
     ev.finish()
 
@@ -698,6 +707,13 @@ def main():
     p.add_argument("--sv-matrices", type=int, default=12)
     p.add_argument("--sv-rate", type=float, default=200.0, help="mean arrivals per second")
     p.add_argument("--sv-n", type=float, nargs=2, default=[1e4, 1e5])
+    p.add_argument("--sv-eigs", type=int, nargs=2, default=[20, 100],
+                   help="eigenvalues per slice request (range)")
+    p.add_argument("--sv-forms", nargs="+", default=["mixed", "fp64"],
+                   choices=["mixed", "fp64"], help="servers to run, in turn")
+    p.add_argument("--no-group", action="store_true",
+                   help="request loop: filter one matrix at a time instead of grouped "
+                        "launches over all matrices")
     p.add_argument("--no-tune", action="store_true",
                    help="run the fused filter kernel on its fixed layout policy (no per-matrix "
                         "layout tuning)")

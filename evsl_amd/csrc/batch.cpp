@@ -352,8 +352,15 @@ struct Slice {
 // p(A~)U = U p(Theta~) + sum_k mu_k D_k with Theta~ the Rayleigh quotients
 // of U, D_1 = R = A~U - U Theta~ (FP64) and
 // D_{k+1} = 2 A~ D_k - D_{k-1} + 2 R T_k(Theta~).  The D recurrence runs in
-// FP32 on the shifted matrix; its rounding scales with ||R||, not ||U||.
-// On exit U holds Z; W is n x m scratch.
+// FP32 on the shifted matrix; its rounding scales with ||R||, not ||U||
+// (relative error growing like k^1.5 eps_fp32).  A column whose residual is
+// large (an unconverged or spurious Ritz vector: ||R|| > kFp64ResidualCut)
+// would carry that much noise into the Rayleigh-Ritz space, where it couples
+// to the converged vectors of nearby Rayleigh quotients; such columns are
+// filtered in FP64 instead (p(A~)u directly).  On exit U holds Z; W is n x m
+// scratch.
+constexpr double kFp64ResidualCut = 1e-5;
+
 int residual_filter(int n, int m, double* U, double* W, const polparams* pol, BlockWork& bw) {
   Context& c = ctx();
   const int deg = pol->deg;
@@ -372,7 +379,7 @@ int residual_filter(int n, int m, double* U, double* W, const polparams* pol, Bl
   if (int e = apply_A_cols(n, m, U, W, bw)) return e;
   EVSL_HIP(evsl_dev::col_dot(n, m, U, n, U, n, part.as<double>(), th2.as<double>(), c.stream));
   EVSL_HIP(evsl_dev::col_dot(n, m, W, n, U, n, part.as<double>(), th.as<double>(), c.stream));
-  std::vector<double> uu(m), ua(m), thv(m), ths(m);
+  std::vector<double> uu(m), ua(m), thv(m), ths(m), rr(m);
   EVSL_HIP(hipMemcpy(uu.data(), th2.p, m * sizeof(double), hipMemcpyDeviceToHost));
   EVSL_HIP(hipMemcpy(ua.data(), th.p, m * sizeof(double), hipMemcpyDeviceToHost));
   for (int j = 0; j < m; ++j) {
@@ -381,75 +388,103 @@ int residual_filter(int n, int m, double* U, double* W, const polparams* pol, Bl
   }
   EVSL_HIP(hipMemcpy(th.p, thv.data(), m * sizeof(double), hipMemcpyHostToDevice));
   EVSL_HIP(evsl_dev::col_resid(n, m, W, n, U, n, th.as<double>(), 1.0 / pol->dd, W, n, c.stream));
-  // p(theta~) per column
-  std::vector<double> pth(m);
-  for (int j = 0; j < m; ++j) evsl::chebxPltd(deg, pol->mu, 1, &ths[j], &pth[j]);
+  // residual norms (relative to the column's norm): the FP32 recurrence for
+  // the small ones, FP64 filtering for the large ones
+  EVSL_HIP(evsl_dev::col_dot(n, m, W, n, W, n, part.as<double>(), th2.as<double>(), c.stream));
+  EVSL_HIP(hipMemcpy(rr.data(), th2.p, m * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<int> small, big;
+  for (int j = 0; j < m; ++j) {
+    if (std::sqrt(std::max(rr[j], 0.0) / uu[j]) > kFp64ResidualCut)
+      big.push_back(j);
+    else
+      small.push_back(j);
+  }
+  // p(theta~) per column (0 for the FP64-filtered columns: W then holds
+  // their whole p(A~)u)
+  std::vector<double> pth(m, 0.0);
+  for (int j : small) evsl::chebxPltd(deg, pol->mu, 1, &ths[j], &pth[j]);
   EVSL_HIP(hipMemcpy(pv.p, pth.data(), m * sizeof(double), hipMemcpyHostToDevice));
-  FilterOp op;
-  if (int e = make_filter_op(pol->cc, pol->dd, true, &op)) return e;
-  // per-launch coefficient tables of every chunk, uploaded once:
-  // g[chunk][k][c] = 2 T_k(theta~_c), k = 1..deg-1; pairs (mu_{k+1}, mu_1)
-  const int nchunk = (m + kBmaxPow - 1) / kBmaxPow;
-  const size_t gstride = (size_t)(deg + 1) * kBmaxPow;
-  std::vector<double> g(gstride * nchunk, 0.0), mp((size_t)2 * (deg + 1), 0.0);
-  for (int j0 = 0, ch = 0; j0 < m; j0 += kBmaxPow, ++ch) {
-    const int b = std::min(kBmaxPow, m - j0);
-    const int B = kernel_width(b);
-    for (int cc = 0; cc < b; ++cc) {
-      const double x = ths[j0 + cc];
-      double tkm1 = 1.0, tk = x;
-      for (int k = 1; k < deg; ++k) {
-        g[gstride * ch + (size_t)k * B + cc] = 2.0 * tk;
-        const double tkp1 = 2.0 * x * tk - tkm1;
-        tkm1 = tk;
-        tk = tkp1;
+  if (!big.empty()) {
+    std::vector<const double*> in;
+    std::vector<double*> out;
+    std::vector<const polparams*> pp(big.size(), pol);
+    for (int j : big) {
+      in.push_back(U + (size_t)j * n);
+      out.push_back(W + (size_t)j * n);
+    }
+    BlockWork bw64;
+    if (int e = filter_columns((int)big.size(), in.data(), out.data(), pp.data(), false, bw64))
+      return e;
+    c.astats.n_refine_fp64_cols += big.size();
+  }
+  const int ms = (int)small.size();
+  if (ms > 0) {
+    FilterOp op;
+    if (int e = make_filter_op(pol->cc, pol->dd, true, &op)) return e;
+    // per-launch coefficient tables of every chunk, uploaded once:
+    // g[chunk][k][c] = 2 T_k(theta~_c), k = 1..deg-1; pairs (mu_{k+1}, mu_1)
+    const int nchunk = (ms + kBmaxPow - 1) / kBmaxPow;
+    const size_t gstride = (size_t)(deg + 1) * kBmaxPow;
+    std::vector<double> g(gstride * nchunk, 0.0), mp((size_t)2 * (deg + 1), 0.0);
+    for (int j0 = 0, ch = 0; j0 < ms; j0 += kBmaxPow, ++ch) {
+      const int b = std::min(kBmaxPow, ms - j0);
+      const int B = kernel_width(b);
+      for (int cc = 0; cc < b; ++cc) {
+        const double x = ths[small[j0 + cc]];
+        double tkm1 = 1.0, tk = x;
+        for (int k = 1; k < deg; ++k) {
+          g[gstride * ch + (size_t)k * B + cc] = 2.0 * tk;
+          const double tkp1 = 2.0 * x * tk - tkm1;
+          tkm1 = tk;
+          tk = tkp1;
+        }
       }
     }
-  }
-  for (int k = 1; k < deg; ++k) {
-    mp[2 * k] = pol->mu[k + 1];
-    mp[2 * k + 1] = pol->mu[1];
-  }
-  DevBuf g_d, mp_d;
-  EVSL_HIP(g_d.alloc(g.size() * sizeof(double)));
-  EVSL_HIP(mp_d.alloc(mp.size() * sizeof(double)));
-  EVSL_HIP(hipMemcpy(g_d.p, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice));
-  EVSL_HIP(hipMemcpy(mp_d.p, mp.data(), mp.size() * sizeof(double), hipMemcpyHostToDevice));
-  for (int j0 = 0, ch = 0; j0 < m; j0 += kBmaxPow, ++ch) {
-    const int b = std::min(kBmaxPow, m - j0);
-    const int B = kernel_width(b);
-    if (int e = bw.ensure(n, B, true, deg)) return e;
-    // R columns -> FP32 block (xk of the first launch = D_1 = R); the same
-    // block is the v0 operand (R) of every launch
-    std::vector<const double*> in(B);
-    std::vector<double*> out(B);
-    for (int cc = 0; cc < B; ++cc) {
-      in[cc] = cc < b ? W + (size_t)(j0 + cc) * n : bw.zero.as<double>();
-      out[cc] = cc < b ? W + (size_t)(j0 + cc) * n : bw.junk.as<double>();
+    for (int k = 1; k < deg; ++k) {
+      mp[2 * k] = pol->mu[k + 1];
+      mp[2 * k + 1] = pol->mu[1];
     }
-    EVSL_HIP(evsl_dev::pack_cols(n, B, col_ptrs(in.data(), B), bw.X0.p, true, c.stream));
-    void* X0 = bw.X0.p;  // R (FP32) = D_1
-    void* bufs[3] = {bw.buf(0, n, B), bw.buf(1, n, B), bw.buf(2, n, B)};
-    void* Yb = bw.Y.p;
-    const double* gch = g_d.as<double>() + gstride * ch;
-    if (deg == 1) {
-      // Y = mu_1 R (W already holds R in FP64)
-      for (int cc = 0; cc < b; ++cc)
-        EVSL_HIP(evsl_dev::scal(n, pol->mu[1], W + (size_t)(j0 + cc) * n, c.stream));
-    } else {
-      for (int k = 1; k < deg; ++k) {
-        void* xkp1 = bufs[(k - 1) % 3];
-        const void* xk = (k == 1) ? X0 : bufs[(k - 2) % 3];
-        const void* xkm1 = (k == 1) ? nullptr : (k == 2 ? X0 : bufs[(k - 3) % 3]);
-        const int yf = evsl_dev::y_flush(k, deg - 1);
-        EVSL_HIP(evsl_dev::cheb_spmm(op.A, B, true, evsl_dev::kRes, xk, xkm1, xkp1, Yb, X0,
-                                     gch + (size_t)k * B, mp_d.as<double>() + 2 * k, op.t2,
-                                     op.cc, k == 1, yf, nullptr, c.stream));
-        // the kCheb streams plus the residual block R
-        c.astats.refine_bytes += (double)c.A.nnz * (4 + c.A.idx_bytes()) + 4.0 * (n + 1) +
-                                 (evsl_dev::cheb_vec_bytes(yf, k == 1, 4) + 4.0) * B * n;
+    DevBuf g_d, mp_d;
+    EVSL_HIP(g_d.alloc(g.size() * sizeof(double)));
+    EVSL_HIP(mp_d.alloc(mp.size() * sizeof(double)));
+    EVSL_HIP(hipMemcpy(g_d.p, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice));
+    EVSL_HIP(hipMemcpy(mp_d.p, mp.data(), mp.size() * sizeof(double), hipMemcpyHostToDevice));
+    for (int j0 = 0, ch = 0; j0 < ms; j0 += kBmaxPow, ++ch) {
+      const int b = std::min(kBmaxPow, ms - j0);
+      const int B = kernel_width(b);
+      if (int e = bw.ensure(n, B, true, deg)) return e;
+      // R columns -> FP32 block (xk of the first launch = D_1 = R); the same
+      // block is the v0 operand (R) of every launch
+      std::vector<const double*> in(B);
+      std::vector<double*> out(B);
+      for (int cc = 0; cc < B; ++cc) {
+        in[cc] = cc < b ? W + (size_t)small[j0 + cc] * n : bw.zero.as<double>();
+        out[cc] = cc < b ? W + (size_t)small[j0 + cc] * n : bw.junk.as<double>();
       }
-      EVSL_HIP(evsl_dev::unpack_cols(n, B, Yb, false, col_ptrs(out.data(), B), c.stream));
+      EVSL_HIP(evsl_dev::pack_cols(n, B, col_ptrs(in.data(), B), bw.X0.p, true, c.stream));
+      void* X0 = bw.X0.p;  // R (FP32) = D_1
+      void* bufs[3] = {bw.buf(0, n, B), bw.buf(1, n, B), bw.buf(2, n, B)};
+      void* Yb = bw.Y.p;
+      const double* gch = g_d.as<double>() + gstride * ch;
+      if (deg == 1) {
+        // Y = mu_1 R (W already holds R in FP64)
+        for (int cc = 0; cc < b; ++cc)
+          EVSL_HIP(evsl_dev::scal(n, pol->mu[1], W + (size_t)small[j0 + cc] * n, c.stream));
+      } else {
+        for (int k = 1; k < deg; ++k) {
+          void* xkp1 = bufs[(k - 1) % 3];
+          const void* xk = (k == 1) ? X0 : bufs[(k - 2) % 3];
+          const void* xkm1 = (k == 1) ? nullptr : (k == 2 ? X0 : bufs[(k - 3) % 3]);
+          const int yf = evsl_dev::y_flush(k, deg - 1);
+          EVSL_HIP(evsl_dev::cheb_spmm(op.A, B, true, evsl_dev::kRes, xk, xkm1, xkp1, Yb, X0,
+                                       gch + (size_t)k * B, mp_d.as<double>() + 2 * k, op.t2,
+                                       op.cc, k == 1, yf, nullptr, c.stream));
+          // the kCheb streams plus the residual block R
+          c.astats.refine_bytes += (double)c.A.nnz * (4 + c.A.idx_bytes()) + 4.0 * (n + 1) +
+                                   (evsl_dev::cheb_vec_bytes(yf, k == 1, 4) + 4.0) * B * n;
+        }
+        EVSL_HIP(evsl_dev::unpack_cols(n, B, Yb, false, col_ptrs(out.data(), B), c.stream));
+      }
     }
   }
   // U <- U p(theta~) + Y
@@ -721,18 +756,19 @@ struct LanczosGroup::Impl {
   DevBuf gt_part, hbuf, scal_d, vrand, parts;
   int kcap = 0, scap = 0;
   BlockWork bw32, bw64;  // the FP32 and the FP64 blocked filter passes
-  // filters the listed slots' columns in[j] -> out[j], one pass per precision
-  int filter(const std::vector<int>& ids, const std::vector<const double*>& in,
-             const std::vector<double*>& out) {
+  std::vector<int> st_cur, act_cur;  // slots of the phase in progress
+  // filters the columns in -> out of this group's matrix, one pass per
+  // precision
+  int filter(const std::vector<LanczosGroup::FilterCol>& cols) {
     for (int pr = 0; pr < 2; ++pr) {
       std::vector<const double*> ii;
       std::vector<double*> oo;
       std::vector<const polparams*> pp;
-      for (size_t j = 0; j < ids.size(); ++j)
-        if (slots[ids[j]]->f32 == (pr == 1)) {
-          ii.push_back(in[j]);
-          oo.push_back(out[j]);
-          pp.push_back(slots[ids[j]]->L.pol);
+      for (const auto& col : cols)
+        if (col.f32 == (pr == 1)) {
+          ii.push_back(col.in);
+          oo.push_back(col.out);
+          pp.push_back(col.pol);
         }
       if (ii.empty()) continue;
       if (int e = filter_columns((int)ii.size(), ii.data(), oo.data(), pp.data(), pr == 1,
@@ -845,7 +881,89 @@ int LanczosGroup::add(const double* intv, int maxit, double tol, const double* v
   return g.ensure(sl.maxit);
 }
 
+int LanczosGroup::start_cols(std::vector<FilterCol>& cols) {
+  Impl& g = *p_;
+  const int n = ctx().n;
+  g.st_cur.clear();
+  for (int s = 0; s < (int)g.slots.size(); ++s)
+    if (g.slots[s]->state == 1) {
+      g.st_cur.push_back(s);
+      Impl::Slot& sl = *g.slots[s];
+      cols.push_back({sl.v0, sl.L.vcol(0, n), sl.L.pol, sl.f32});
+    }
+  return 0;
+}
+
+// the new slices' filtered start vectors: normalise, make the slices active
+int LanczosGroup::start_done() {
+  Impl& g = *p_;
+  Context& c = ctx();
+  const int n = c.n;
+  if (g.st_cur.empty()) return 0;
+  double* sc_d = g.scal_d.as<double>();
+  for (int s : g.st_cur) {
+    double* v = g.slots[s]->L.vcol(0, n);
+    EVSL_HIP(evsl_dev::dot_partial(n, v, v, g.part_of(s), c.stream));
+    EVSL_HIP(evsl_dev::sum_scalar(g.part_of(s), evsl_dev::kRedBlocks, sc_d + 4 * s, c.stream));
+  }
+  std::vector<double> nv(4 * g.slots.size());
+  EVSL_HIP(hipMemcpy(nv.data(), sc_d, nv.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int s : g.st_cur) {
+    EVSL_HIP(evsl_dev::scal(n, 1.0 / std::sqrt(nv[4 * s]), g.slots[s]->L.vcol(0, n), c.stream));
+    g.slots[s]->state = 2;
+  }
+  g.st_cur.clear();
+  return 0;
+}
+
+int LanczosGroup::step_cols(std::vector<FilterCol>& cols) {
+  Impl& g = *p_;
+  const int n = ctx().n;
+  g.act_cur.clear();
+  for (int s = 0; s < (int)g.slots.size(); ++s)
+    if (g.slots[s]->state == 2) {
+      g.act_cur.push_back(s);
+      Impl::Slot& sl = *g.slots[s];
+      cols.push_back({sl.L.vcol(sl.k, n), sl.L.vcol(sl.k + 1, n), sl.L.pol, sl.f32});
+    }
+  return 0;
+}
+
 int LanczosGroup::step() {
+  Impl& g = *p_;
+  Context& c = ctx();
+  if (!g.ev[0])
+    for (auto& e : g.ev) EVSL_HIP(hipEventCreate(&e));
+  auto elapsed = [](hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess)
+      return 0.0;
+    return ms * 1e-3;
+  };
+  // new slices: filter their start vectors together, normalise
+  std::vector<FilterCol> cols;
+  if (int e = start_cols(cols)) return e;
+  if (!cols.empty()) {
+    EVSL_HIP(hipEventRecord(g.ev[0], c.stream));
+    if (int e = g.filter(cols)) return e;
+    EVSL_HIP(hipEventRecord(g.ev[1], c.stream));
+    c.astats.t_filter += elapsed(g.ev[0], g.ev[1]);
+    if (int e = start_done()) return e;
+  }
+  // one blocked filter pass for all active slices
+  cols.clear();
+  if (int e = step_cols(cols)) return e;
+  if (cols.empty()) return 0;
+  EVSL_HIP(hipEventRecord(g.ev[0], c.stream));
+  if (int e = g.filter(cols)) return e;
+  EVSL_HIP(hipEventRecord(g.ev[1], c.stream));
+  c.astats.t_filter += elapsed(g.ev[0], g.ev[1]);
+  return step_done();
+}
+
+// the rest of the step once the active slices' vectors are filtered:
+// recurrence, re-orthogonalisation, convergence tests, results
+int LanczosGroup::step_done() {
   Impl& g = *p_;
   Context& c = ctx();
   const int n = c.n;
@@ -861,54 +979,11 @@ int LanczosGroup::step() {
   const double eta = 1.0 / std::sqrt(2.0);
   const int nb_rows = (n + evsl_dev::kWG - 1) / evsl_dev::kWG;
   double* sc_d = g.scal_d.as<double>();
-  // ---- new slices: filter their start vectors together, normalize ---------
-  {
-    std::vector<int> st;
-    for (int s = 0; s < (int)g.slots.size(); ++s)
-      if (g.slots[s]->state == 1) st.push_back(s);
-    if (!st.empty()) {
-      std::vector<const double*> in;
-      std::vector<double*> out;
-      for (int s : st) {
-        in.push_back(g.slots[s]->v0);
-        out.push_back(g.slots[s]->L.vcol(0, n));
-      }
-      EVSL_HIP(hipEventRecord(g.ev[0], c.stream));
-      if (int e = g.filter(st, in, out)) return e;
-      EVSL_HIP(hipEventRecord(g.ev[1], c.stream));
-      c.astats.t_filter += elapsed(g.ev[0], g.ev[1]);
-      for (int s : st) {
-        double* v = g.slots[s]->L.vcol(0, n);
-        EVSL_HIP(evsl_dev::dot_partial(n, v, v, g.part_of(s), c.stream));
-        EVSL_HIP(evsl_dev::sum_scalar(g.part_of(s), evsl_dev::kRedBlocks, sc_d + 4 * s, c.stream));
-      }
-      std::vector<double> nv(4 * g.slots.size());
-      EVSL_HIP(hipMemcpy(nv.data(), sc_d, nv.size() * sizeof(double), hipMemcpyDeviceToHost));
-      for (int s : st) {
-        EVSL_HIP(evsl_dev::scal(n, 1.0 / std::sqrt(nv[4 * s]), g.slots[s]->L.vcol(0, n),
-                                c.stream));
-        g.slots[s]->state = 2;
-      }
-    }
-  }
-  std::vector<int> act;
-  for (int s = 0; s < (int)g.slots.size(); ++s)
-    if (g.slots[s]->state == 2) act.push_back(s);
+  const std::vector<int> act = g.act_cur;
   if (act.empty()) return 0;
   const int b = (int)act.size();
-  // ---- one blocked filter pass for all active slices -------------------------
-  {
-    std::vector<const double*> in(b);
-    std::vector<double*> out(b);
-    for (int j = 0; j < b; ++j) {
-      Impl::Slot& sl = *g.slots[act[j]];
-      in[j] = sl.L.vcol(sl.k, n);
-      out[j] = sl.L.vcol(sl.k + 1, n);
-    }
-    EVSL_HIP(hipEventRecord(g.ev[0], c.stream));
-    if (int e = g.filter(act, in, out)) return e;
-    EVSL_HIP(hipEventRecord(g.ev[1], c.stream));
-  }
+  EVSL_HIP(hipEventRecord(g.ev[1], c.stream));
+
   // ---- recurrence: vnew -= beta vold; alpha; vnew -= alpha v; ||vnew||^2 -----
   for (int s : act) {
     Impl::Slot& sl = *g.slots[s];
@@ -999,7 +1074,6 @@ int LanczosGroup::step() {
     L.eT[k] = L.beta;
   }
   EVSL_HIP(hipEventRecord(g.ev[2], c.stream));
-  c.astats.t_filter += elapsed(g.ev[0], g.ev[1]);
   c.astats.t_reorth += elapsed(g.ev[1], g.ev[2]);
   c.astats.n_lanczos += b;
   // ---- convergence tests (restricted trace, every 20 steps from 30) ---------

@@ -187,6 +187,35 @@ struct KpmGroupEntry {
   int n, blk0;  // rows; first global row block of this matrix
   int k0, pad;
 };
+// One column block of a grouped ChebAv launch (a device table entry): its
+// matrix's CSR (16-bit column offsets jd when the launch's instance reads
+// them), values (FP32 shifted copy or FP64) and row-block table for the
+// launch's lane layout, its start block x0 (n x B interleaved), ring of three
+// blocks (T_k written to slot (k-1) % 3), FP64 output block y, coefficient
+// table mus (row d: the B columns' mu_d, zero beyond a column's degree), the
+// recurrence factors (t1 at k = 1, t2 after; cc the shift left to the
+// epilogue) and the block's series degree deg (its last launch flushes y).
+struct ChebGroupEntry {
+  const int* ia;
+  const int* ja;
+  const short* jd;
+  const void* a;
+  const int* rbp;
+  const void* x0;
+  void* ring;
+  double* y;
+  const double* mus;
+  double t1, t2, cc;
+  int n, blk0, deg, pad;
+};
+// Step k of a grouped ChebAv launch over the first `total` row blocks of the
+// entries of one (precision, width B in 1..8, index type) class; owner[gb]
+// is the entry of global row block gb.  Entries are ordered by decreasing
+// degree, so the launch at step k covers the prefix of entries with
+// deg >= k.  Lane layout: the fixed policy's for (B, precision, kCheb).
+hipError_t cheb_group_step(bool fp32, int B, bool i16, const ChebGroupEntry* tab,
+                           const int* owner, int total, int k, hipStream_t st);
+
 // The partial sums of one grouped matrix: nblk block partials (B per block)
 // summed as sum_block_partials does (G first-level blocks into scratch, then
 // one; or directly when G == 1) into dots + (k+1)*B.

@@ -98,6 +98,9 @@ struct Context {
   int mixed_min_deg = 32;    // mixed form: filters of degree outside
   int mixed_max_deg = 20000; // [mixed_min_deg, mixed_max_deg] run in FP64
   bool spmm_tuning = true;   // filter launches go through the matrix's layout tuner
+  // the request loop filters the slices of all its matrices in grouped
+  // launches (else one matrix at a time)
+  bool grouped_filter = true;
   // scratch for reductions
   DevBuf red_part, red_scalars;
   // pinned host scalars for device->host reads
@@ -209,13 +212,62 @@ class LanczosGroup {
   LanczosGroup& operator=(const LanczosGroup&) = delete;
   int add(const double* intv, int maxit, double tol, const double* v0, const polparams* pol,
           SliceResult* out);
+  // one Lanczos step of every slice (its filter passes on this matrix)
   int step();
   int active() const;
+  // The same step in phases, for a caller that filters the columns of
+  // several groups together (grouped launches over their matrices).  Each
+  // phase runs with this group's matrix as the context's operator; the
+  // caller filters the columns a *_cols phase appends (out = p(A) in) before
+  // the matching *_done phase, and accounts the filter time.
+  struct FilterCol {
+    const double* in;
+    double* out;
+    const polparams* pol;
+    bool f32;
+  };
+  int start_cols(std::vector<FilterCol>& cols);  // start vectors of new slices
+  int start_done();                              // normalise them, activate
+  int step_cols(std::vector<FilterCol>& cols);   // the active slices' vectors
+  int step_done();  // recurrence, re-orthogonalisation, tests, results
 
  private:
   struct Impl;
   std::unique_ptr<Impl> p_;
 };
+// ---- grouped filter passes over several matrices (grouped.cpp) -------------
+// The columns of one matrix at one precision: op from make_filter_op with
+// that matrix bound (its pointers and sizes, op.A.n and op.A.nnz, stay valid
+// once unbound), and the columns out = p(A) in (every col.f32 == op.f32;
+// same cc, dd), each an op.A.n-vector.
+struct GroupedCols {
+  FilterOp op;
+  std::vector<LanczosGroup::FilterCol> cols;
+};
+// device work of grouped_filter, reused from call to call
+struct GroupedWork {
+  struct PolKey {
+    const polparams* pol;
+    const double* mu;
+    int deg;
+    bool operator==(const PolKey& o) const { return pol == o.pol && mu == o.mu && deg == o.deg; }
+    bool operator!=(const PolKey& o) const { return !(*this == o); }
+  };
+  struct Bufs {
+    DevBuf X0, ring, Y, mus;
+    std::vector<PolKey> mus_key;
+    int mus_deg = -1;
+  };
+  std::vector<std::unique_ptr<Bufs>> bufs;
+  DevBuf d_tab, d_owner;
+  std::vector<int> owner_h;
+};
+// Filters every part's columns with grouped launches: one launch per
+// Chebyshev degree and class of column blocks (precision, width, index
+// type) covers the row blocks of all matrices whose series has not ended.
+// Each column gets the bits of its one-matrix filter.
+int grouped_filter(const std::vector<GroupedCols>& parts, GroupedWork& W);
+
 // KPM moments and eigenvalue count from raw[k] = sum over the nvec vectors of
 // v^T T_k(A~) v (k >= 1), as kpmdos finishes them
 void kpm_moments(int Mdeg, int damping, int nvec, const double* intv, int n, const double* raw,

@@ -67,6 +67,41 @@ __global__ __launch_bounds__(kWG) void k_kpm_group(const KpmGroupEntry* __restri
       k == 0 ? e.t0 : e.t, e.cc, 0, 0, e.part);
 }
 
+// the y-flush schedule of evsl_dev::y_flush, for the grouped launch (each
+// entry flushes by its own series degree m)
+__device__ __forceinline__ int y_flush_dev(int k, int m) {
+  if (k == 1) return 1;
+  const int r = (k - 1) % 3;
+  if (r == 0) return 3;
+  return k == m ? r : 0;
+}
+
+// ChebAv steps of several column blocks (of different matrices) in one
+// launch: global row block gb (after the XCD remap) belongs to entry
+// owner[gb]; the block runs the one-matrix kCheb step at k with that entry's
+// operands, in the lane layout of the fixed policy, so every column gets the
+// bits of its one-matrix filter.
+template <typename VT, int B, bool I16>
+__global__ __launch_bounds__(kWG) void k_cheb_group(const ChebGroupEntry* __restrict__ tab,
+                                                    const int* __restrict__ owner, int total,
+                                                    int k) {
+  extern __shared__ int s_dyn[];  // the block's row pointers when R > 1
+  using P = SpmmPolicy<VT, B, kCheb>;
+  const int gb = xcd_remap(blockIdx.x, total);
+  const ChebGroupEntry& e = tab[owner[gb]];
+  const size_t nb = (size_t)e.n * B;
+  const VT* x0 = static_cast<const VT*>(e.x0);
+  VT* ring = static_cast<VT*>(e.ring);
+  VT* xkp1 = ring + (size_t)((k - 1) % 3) * nb;
+  const VT* xk = k == 1 ? x0 : ring + (size_t)((k - 2) % 3) * nb;
+  const VT* xkm1 = k == 1 ? nullptr : (k == 2 ? x0 : ring + (size_t)((k - 3) % 3) * nb);
+  cheb_spmm_block<VT, B, P::V, kCheb, true, I16, false>(
+      gb - e.blk0, e.ia, e.ja, e.jd, static_cast<const VT*>(e.a),
+      reinterpret_cast<const int2*>(e.rbp), xk, xkm1, xkp1, e.y, nullptr,
+      e.mus + (size_t)k * B, e.mus, k == 1 ? e.t1 : e.t2, e.cc, k == 1, y_flush_dev(k, e.deg),
+      nullptr, P::R > 1 ? s_dyn : nullptr);
+}
+
 // out[c] = sum over blocks b in [b0, b1) of part[b*B + c] for block `bid` of
 // `nb` (the range split evenly over the nb blocks; out = scratch of nb * B
 // values when nb > 1)
@@ -799,6 +834,44 @@ __global__ __launch_bounds__(kWG) void k_stream_copy(size_t n16, const u32x4* __
 // ---------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------
+hipError_t cheb_group_step(bool fp32, int B, bool i16, const ChebGroupEntry* tab,
+                           const int* owner, int total, int k, hipStream_t st) {
+  if (total <= 0) return hipSuccess;
+  if (k < 1) return hipErrorInvalidValue;
+  auto go = [&](auto vt_tag, auto b_tag, auto i16_tag) {
+    using VT = decltype(vt_tag);
+    constexpr int BB = decltype(b_tag)::value;
+    constexpr bool I = decltype(i16_tag)::value;
+    const size_t dyn =
+        SpmmPolicy<VT, BB, kCheb>::R > 1 ? (size_t)(kRowsCap + 1) * sizeof(int) : 0;
+    hipLaunchKernelGGL((k_cheb_group<VT, BB, I>), dim3(total), dim3(kWG), dyn, st, tab, owner,
+                       total, k);
+    return hipGetLastError();
+  };
+  auto width = [&](auto vt_tag, auto i16_tag) -> hipError_t {
+    switch (B) {
+#define EVSL_GROUP_CASE(BB) \
+  case BB:                  \
+    return go(vt_tag, std::integral_constant<int, BB>{}, i16_tag);
+      EVSL_GROUP_CASE(1)
+      EVSL_GROUP_CASE(2)
+      EVSL_GROUP_CASE(3)
+      EVSL_GROUP_CASE(4)
+      EVSL_GROUP_CASE(5)
+      EVSL_GROUP_CASE(6)
+      EVSL_GROUP_CASE(7)
+      EVSL_GROUP_CASE(8)
+#undef EVSL_GROUP_CASE
+      default:
+        return hipErrorInvalidValue;
+    }
+  };
+  using T16 = std::integral_constant<bool, true>;
+  using T32 = std::integral_constant<bool, false>;
+  if (fp32) return i16 ? width(0.0f, T16{}) : width(0.0f, T32{});
+  return i16 ? width(0.0, T16{}) : width(0.0, T32{});
+}
+
 hipError_t stream_copy(size_t bytes, const void* src, void* dst, hipStream_t st) {
   const size_t n16 = bytes / 16;
   if (n16 == 0) return hipSuccess;

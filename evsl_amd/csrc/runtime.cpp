@@ -550,6 +550,10 @@ int evsl_amd_set_refine_guard(double guard) {
   ctx().refine_guard = guard;
   return 0;
 }
+int evsl_amd_set_grouped_filter(int on) {
+  ctx().grouped_filter = on != 0;
+  return 0;
+}
 int evsl_amd_set_refine_widen(double widen) {
   if (!(widen >= 0.0 && widen <= 1e6)) return EVSL_ERR_ARG;
   ctx().refine_widen = widen;

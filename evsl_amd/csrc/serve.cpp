@@ -105,6 +105,12 @@ struct evslAmdServer {
   std::deque<int> queue;
   long long kg = 0;  // grouped KPM launches so far (DOS entries start at k0 = kg - k)
   DevBuf d_tab, d_owner, d_sums, d_owner1, d_first1, d_owner2, x64, nrm_part, nrm2;
+  GroupedWork gw;  // the slices' grouped filter passes
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  ~evslAmdServer() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
 };
 
 namespace {
@@ -197,6 +203,49 @@ int admit(evslAmdServer* s) {
       if (int e = dos_round(s, q)) return e;
     }
   }
+  return 0;
+}
+
+// One filter pass of the slices of every listed matrix in grouped launches:
+// the start vectors of slices added since the last step (start), or the
+// current Lanczos vectors of the active slices; then, for a start pass, the
+// new slices are normalised and join.
+int grouped_pass(evslAmdServer* s, const std::vector<int>& live, bool start) {
+  Context& c = ctx();
+  std::vector<GroupedCols> parts;
+  for (int mi : live) {
+    SrvMatrix& m = *s->mats[mi];
+    Binding bind(m.M);
+    std::vector<LanczosGroup::FilterCol> cols;
+    if (int e = start ? m.grp->start_cols(cols) : m.grp->step_cols(cols)) return e;
+    for (int pr = 0; pr < 2; ++pr) {
+      GroupedCols gc;
+      for (const auto& col : cols)
+        if (col.f32 == (pr == 1)) gc.cols.push_back(col);
+      if (gc.cols.empty()) continue;
+      // (the operator's view carries this matrix's sizes; inside the
+      // binding m.M holds the context's previous matrix)
+      const polparams* p0 = gc.cols[0].pol;
+      if (int e = make_filter_op(p0->cc, p0->dd, pr == 1, &gc.op)) return e;
+      parts.push_back(std::move(gc));
+    }
+  }
+  if (!parts.empty()) {
+    if (!s->ev[0])
+      for (auto& e : s->ev) EVSL_HIP(hipEventCreate(&e));
+    EVSL_HIP(hipEventRecord(s->ev[0], c.stream));
+    if (int e = grouped_filter(parts, s->gw)) return e;
+    EVSL_HIP(hipEventRecord(s->ev[1], c.stream));
+    float ms = 0.f;
+    EVSL_HIP(hipEventSynchronize(s->ev[1]));
+    EVSL_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+    c.astats.t_filter += ms * 1e-3;
+  }
+  if (start)
+    for (int mi : live) {
+      Binding bind(s->mats[mi]->M);
+      if (int e = s->mats[mi]->grp->start_done()) return e;
+    }
   return 0;
 }
 
@@ -439,20 +488,32 @@ int evsl_amd_server_step(evslAmdServer* s) {
   if (int e = ensure_started()) return e;
   if (!s) return EVSL_ERR_ARG;
   if (int e = admit(s)) return e;
-  for (int mi = 0; mi < (int)s->mats.size(); ++mi) {
-    SrvMatrix& m = *s->mats[mi];
-    if (!m.grp || m.grp->active() == 0) continue;
-    {
-      Binding bind(m.M);
-      if (int e = m.grp->step()) return e;
+  std::vector<int> live;
+  for (int mi = 0; mi < (int)s->mats.size(); ++mi)
+    if (s->mats[mi]->grp && s->mats[mi]->grp->active() > 0) live.push_back(mi);
+  if (ctx().grouped_filter) {
+    // the filter passes of every matrix's slices in grouped launches (new
+    // slices' start vectors, then every active slice's Lanczos vector), the
+    // rest of the step per matrix
+    if (int e = grouped_pass(s, live, true)) return e;
+    if (int e = grouped_pass(s, live, false)) return e;
+    for (int mi : live) {
+      Binding bind(s->mats[mi]->M);
+      if (int e = s->mats[mi]->grp->step_done()) return e;
     }
+  } else {
+    for (int mi : live) {
+      Binding bind(s->mats[mi]->M);
+      if (int e = s->mats[mi]->grp->step()) return e;
+    }
+  }
+  for (int mi : live)
     for (auto& r : s->reqs)
       if (r->kind == 0 && r->state == 1 && r->mat == mi && r->res.done) {
         r->err = r->res.err;
         finish(s, *r);
         r->t_done = r->res.t_done;
       }
-  }
   if (int e = dos_steps(s)) return e;
   int left = 0;
   for (auto& r : s->reqs) left += r->state != 2;

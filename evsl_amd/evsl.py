@@ -43,6 +43,8 @@ EXPORTS = [
     "evsl_amd_last_error", "evsl_amd_stats_size", "kpmdos_ex", "evsl_amd_bench_filter",
     "ChebLanNr_batch", "evsl_amd_read_coo_mm", "evsl_amd_bench_copy",
     "evsl_amd_spmm_vec", "evsl_amd_spmm_index_bytes", "evsl_amd_cheb_av_cols",
+    "evsl_amd_set_grouped_filter",
+
     "evsl_amd_spmm_streaming", "kpmdos_batch",
     "evsl_amd_server_create", "evsl_amd_server_destroy", "evsl_amd_server_add_matrix",
     "evsl_amd_server_submit_slice", "evsl_amd_server_submit_dos", "evsl_amd_server_step",
@@ -60,7 +62,7 @@ class AmdStats(C.Structure):
                 ("reorth_bytes", C.c_double), ("n_refine_cols", C.c_size_t),
                 ("n_refine_extra", C.c_size_t), ("refine_bytes", C.c_double),
                 ("n_refine_degcols", C.c_size_t), ("n_fp64_slices", C.c_size_t),
-                ("n_fp64_reruns", C.c_size_t)]
+                ("n_fp64_reruns", C.c_size_t), ("n_refine_fp64_cols", C.c_size_t)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -151,6 +153,7 @@ class EVSL:
             "evsl_amd_server_stats": (I, [VP, ip, ip, C.POINTER(C.c_size_t),
                                           C.POINTER(C.c_size_t)]),
             "evsl_amd_cheb_av_cols": (I, [I, pp, dp, dp, I]),
+            "evsl_amd_set_grouped_filter": (I, [I]),
             "ChebLanNr_batch": (I, [I, dp, I, D, dp, pp, ip, ddp, ddp, ddp, VP]),
         }
         for name, (res, args) in sig.items():

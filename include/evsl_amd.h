@@ -40,6 +40,8 @@ typedef struct _evslAmdStats {
                          /* filter)                                        */
   size_t n_fp64_reruns;  /* MIXED: slices redone in FP64 after a stalled   */
                          /* refinement                                     */
+  size_t n_refine_fp64_cols; /* refinement columns filtered in FP64 (their */
+                             /* residual too large for the FP32 recurrence) */
 } evslAmdStats;
 
 #ifdef __cplusplus
@@ -134,10 +136,13 @@ int evsl_amd_cheb_av_cols(int ncols, const polparams *pols, const double *X, dou
  * requests (kpmdos_ex with seeded device vectors; count over [a, b]) are
  * queued by submit; each evsl_amd_server_step admits queued requests in
  * arrival order while their projected memory (n x (maxit+2) x 8 bytes for a
- * slice) fits mem_budget, advances every running slice of each matrix by one
- * filtered Lanczos step in one blocked pass, and every running DOS request by
- * up to 64 Chebyshev degrees in grouped launches.  A request's result does
- * not depend on the other requests sharing its steps.  step returns the
+ * slice) fits mem_budget, advances every running slice by one filtered
+ * Lanczos step -- the filter passes of all matrices' slices in grouped
+ * launches (one launch per degree covers the row blocks of every matrix whose
+ * filters have not reached it; evsl_amd_set_grouped_filter(0) filters one
+ * matrix at a time instead) -- and every running DOS request by up to 64
+ * Chebyshev degrees in grouped launches.  A request's result does not depend
+ * on the other requests sharing its steps (bit for bit).  step returns the
  * number of requests not yet done (0 when idle) or an error.  status returns
  * 0 queued, 1 running, 2 done, with evsl_timer() stamps.  Results: slice ->
  * lam, res host (evsl_Free), Y device (evsl_Free_device), ownership passes
@@ -158,6 +163,9 @@ int evsl_amd_server_slice_result(evslAmdServer *s, int req, int *nev, double **l
 int evsl_amd_server_dos_result(evslAmdServer *s, int req, double *mu, double *ecnt);
 int evsl_amd_server_stats(evslAmdServer *s, int *queued, int *running, size_t *mem_in_use,
                           size_t *mem_peak);
+/* the request loop's slice filters: grouped launches over all matrices (1,
+ * the default) or one matrix at a time (0); results are the same bits */
+int evsl_amd_set_grouped_filter(int on);
 
 /* Matrix Market coordinate reader of the reference drivers
  * (TESTS/COMMON/io.c:115-217, read_coo_MM): real/integer, general/symmetric

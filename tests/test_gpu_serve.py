@@ -144,3 +144,37 @@ def test_server_admission_by_memory(ev):
     assert stats["mem_in_use"] == 0 and stats["running"] == 0 and stats["queued"] == 0
     for lam, res in out:
         assert len(lam) > 0 and np.all(res < TOL)
+
+
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_MIXED])
+def test_server_grouped_launches_equal_per_matrix_filters(ev, prec):
+    """The request loop filters the slices of all its matrices in grouped
+    launches (one launch per degree and class of column blocks: precision,
+    width, 16- or 32-bit column offsets).  The same stream with grouping off
+    (one matrix at a time, the matrix's own blocked filter) gives the same
+    bits: five slices share the 2-D Laplacian (a 5-column block, filters of
+    different degrees), and a random symmetric matrix of 40,000 rows takes
+    the 32-bit-index class."""
+    import scipy.sparse.linalg as sla
+    P = pool()
+    R = M.random_symmetric(40000, 8, 16, seed=3)
+    lo, hi = M.gershgorin(R)
+    top = np.sort(sla.eigsh(R.to_scipy(), k=24, which="LA", return_eigenvectors=False))
+    P.append((R, lo, hi))
+    reqs = [("slice", 0, 0.3 + 0.08 * i, 0.38 + 0.08 * i, 40 + i) for i in range(5)]
+    reqs += [("slice", 1, 0.6, 0.9, 12), ("slice", 3, 0.5 * (top[2] + top[3]),
+                                          0.5 * (top[-3] + top[-2]), 16),
+             ("dos", 3, lo, hi, 24)]
+    try:
+        ev.lib.evsl_amd_set_grouped_filter(1)
+        grouped, _ = run(ev, P, reqs, prec)
+        ev.lib.evsl_amd_set_grouped_filter(0)
+        single, _ = run(ev, P, reqs, prec)
+    finally:
+        ev.lib.evsl_amd_set_grouped_filter(1)
+    for r, a, b in zip(reqs, grouped, single):
+        assert np.array_equal(a[0], b[0]), (prec, r)
+        assert np.array_equal(a[1], b[1]), (prec, r)
+        if r[0] == "slice":
+            assert len(a[0]) > 0, r
+            assert np.all(a[1] < TOL)
