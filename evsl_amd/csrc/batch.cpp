@@ -23,6 +23,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "internal.h"
@@ -1508,6 +1509,128 @@ int evsl_amd_cheb_av_cols(int ncols, const polparams* pols, const double* X, dou
     return e;
   if (!ydev)
     EVSL_HIP(hipMemcpyAsync(Y, yd, tot * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  EVSL_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+// evsl_amd_cheb_av_cols with every work buffer in the library's static
+// device scratch (evsl_dev::kNoAllocScratchBytes per GPU): no device memory
+// is allocated or freed, for callers that cannot allocate.  X and Y must be
+// device memory; the columns run in blocks of at most 32 whose interleaved
+// input, recurrence ring (three blocks), FP64 output block and coefficient
+// table fit the scratch.  The fused kernel runs the fixed layout policy (no
+// layout tuning, whose timing events are resources), which gives the same
+// bits as any tuned layout, so the result equals evsl_amd_cheb_av_cols's
+// bit for bit.  Calls that overlap on one GPU share its scratch and race.
+int evsl_amd_cheb_av_cols_noalloc(int ncols, const polparams* pols, const double* X, double* Y,
+                                  int prec) {
+  clear_error();
+  Context& c = ctx();
+  if (!c.started || c.op != 1) {
+    set_error_msg("evsl_amd_cheb_av_cols_noalloc: no device matrix (EVSLStart, SetAMatrix)");
+    return EVSL_ERR_NOMAT;
+  }
+  if (ncols <= 0 || !pols || !X || !Y || (prec != EVSL_PREC_FP64 && prec != EVSL_PREC_MIXED)) {
+    set_error_msg("evsl_amd_cheb_av_cols_noalloc: invalid arguments");
+    return EVSL_ERR_ARG;
+  }
+  for (int j = 0; j < ncols; ++j)
+    if (!pols[j].mu || pols[j].deg < 0 || pols[j].cc != pols[0].cc ||
+        pols[j].dd != pols[0].dd) {
+      set_error_msg("evsl_amd_cheb_av_cols_noalloc: every filter needs coefficients and the "
+                    "same cc, dd");
+      return EVSL_ERR_ARG;
+    }
+  for (const void* p : {(const void*)X, (const void*)Y}) {
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess ||
+        !(at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged)) {
+      (void)hipGetLastError();
+      set_error_msg("evsl_amd_cheb_av_cols_noalloc: X and Y must be device memory");
+      return EVSL_ERR_ARG;
+    }
+  }
+  const bool f32 = prec == EVSL_PREC_MIXED;
+  if (!filter_op_ready(f32)) {
+    set_error_msg("evsl_amd_cheb_av_cols_noalloc: the matrix's FP32 copy does not exist yet "
+                  "(one mixed-precision call on this matrix creates it)");
+    return EVSL_ERR_ARG;
+  }
+  void* base = nullptr;
+  if (hipError_t he = evsl_dev::noalloc_scratch(&base); he != hipSuccess) {
+    set_error("evsl_amd_cheb_av_cols_noalloc: scratch address", he);
+    return EVSL_ERR_DEVICE;
+  }
+  const size_t n = c.n, vs = f32 ? 4 : 8, cap = evsl_dev::kNoAllocScratchBytes;
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  // scratch bytes of a block of B columns filtered to degree md
+  auto need = [&](int B, int md) {
+    return up((size_t)(md + 1) * B * sizeof(double)) + 4 * up(n * B * vs) +
+           up(n * B * sizeof(double));
+  };
+  FilterOp op;
+  bool have_op = false;
+  for (int j0 = 0; j0 < ncols;) {
+    int B = 0, md = 0;
+    for (int w : {32, 16, 8, 7, 6, 5, 4, 3, 2, 1}) {
+      if (w > ncols - j0) continue;
+      int d = 0;
+      for (int j = j0; j < j0 + w; ++j) d = std::max(d, pols[j].deg);
+      if (need(w, d) <= cap) {
+        B = w;
+        md = d;
+        break;
+      }
+    }
+    if (B == 0) {
+      set_error_msg("evsl_amd_cheb_av_cols_noalloc: one column needs more than the scratch's " +
+                    std::to_string(cap >> 20) + " MiB (n x (4 x vector size + 8) bytes plus " +
+                    "(deg + 1) x 8)");
+      return EVSL_ERR_ARG;
+    }
+    const double* in[evsl_dev::kMaxCols];
+    double* out[evsl_dev::kMaxCols];
+    for (int j = 0; j < B; ++j) {
+      in[j] = X + (size_t)(j0 + j) * n;
+      out[j] = Y + (size_t)(j0 + j) * n;
+    }
+    if (md == 0) {  // degree-0 filters: y = mu0 x
+      for (int j = 0; j < B; ++j)
+        EVSL_HIP(evsl_dev::scal_copy(n, pols[j0 + j].mu[0], in[j], out[j], c.stream));
+      j0 += B;
+      continue;
+    }
+    if (!have_op) {
+      if (int e = make_filter_op(pols[0].cc, pols[0].dd, f32, &op)) return e;
+      op.A.tune = nullptr;  // fixed layout policy
+      have_op = true;
+    }
+    char* p = static_cast<char*>(base);
+    double* tab = reinterpret_cast<double*>(p);
+    p += up((size_t)(md + 1) * B * sizeof(double));
+    void* X0 = p;
+    p += up(n * B * vs);
+    void* bufs[3];
+    for (int i = 0; i < 3; ++i) {
+      bufs[i] = p;
+      p += up(n * B * vs);
+    }
+    void* Yb = p;
+    // coefficient table: row d holds mu_d of each column, zero beyond a
+    // column's degree (the column's sum is then the one of its own degree)
+    EVSL_HIP(evsl_dev::fill((size_t)(md + 1) * B, 0.0, tab, c.stream));
+    for (int j = 0; j < B; ++j)
+      EVSL_HIP(hipMemcpy2DAsync(tab + j, B * sizeof(double), pols[j0 + j].mu, sizeof(double),
+                                sizeof(double), pols[j0 + j].deg + 1, hipMemcpyHostToDevice,
+                                c.stream));
+    EVSL_HIP(evsl_dev::pack_cols(n, B, col_ptrs(in, B), X0, f32, c.stream));
+    if (int e = block_filter(B, op, tab, md, X0, bufs, Yb)) return e;
+    EVSL_HIP(evsl_dev::unpack_cols(n, B, Yb, false, col_ptrs(out, B), c.stream));
+    // the next block reuses the scratch: the stream orders its launches
+    // after these; the host's coefficient arrays are read by then
+    EVSL_HIP(hipStreamSynchronize(c.stream));
+    j0 += B;
+  }
   EVSL_HIP(hipStreamSynchronize(c.stream));
   return 0;
 }

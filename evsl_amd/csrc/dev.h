@@ -152,6 +152,13 @@ inline double cheb_vec_bytes(int yflush, bool first, int vs) {
   return first ? 2.0 * vs + 8.0 : 3.0 * vs + (yflush ? 16.0 : 0.0);
 }
 
+// Device scratch of the allocation-free filter entry point (noalloc.hip): a
+// file-scope device array of kNoAllocScratchBytes per GPU, held until the
+// process ends.  noalloc_scratch returns its address on the current device
+// (looked up once per device).
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+hipError_t noalloc_scratch(void** ptr);
+
 // dst = src for bytes / 16 16-byte words (the bandwidth reference kernel)
 hipError_t stream_copy(size_t bytes, const void* src, void* dst, hipStream_t st);
 

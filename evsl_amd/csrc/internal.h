@@ -109,6 +109,8 @@ struct Context {
 };
 
 Context& ctx();
+// forget the previous call's error message
+void clear_error();
 int ensure_started();
 // make sure the FP32 copy of the matrix values exists
 int ensure_a32();
@@ -167,6 +169,10 @@ struct FilterOp {
   bool f32 = false;
 };
 int make_filter_op(double cc, double dd, bool f32, FilterOp* op);
+// whether make_filter_op(.., f32, ..) on the current device matrix runs
+// without allocating: always in FP64; in FP32 once the matrix's FP32 copy
+// exists (a shifted copy is recomputed in place for another cc, dd)
+bool filter_op_ready(bool f32);
 
 // ---- filter application (lanczos.cpp) -----------------------------------
 struct ChebWork {

@@ -108,6 +108,8 @@ Context& ctx() { return g_ctx; }
 
 // Called first by every device entry point: the error message then belongs
 // to the first failure of the current call.
+void clear_error() { g_err.clear(); }
+
 int ensure_started() {
   Context& c = g_ctx;
   g_err.clear();
@@ -274,6 +276,13 @@ int ensure_a32_shifted(double cc, double scale, bool* ok) {
   return 0;
 }
 
+bool filter_op_ready(bool f32) {
+  const Context& c = g_ctx;
+  if (c.op != 1) return false;
+  if (!f32) return true;
+  return c.A.full_diag ? c.A.a32s != nullptr : c.A.a32 != nullptr;
+}
+
 int make_filter_op(double cc, double dd, bool f32, FilterOp* op) {
   Context& c = g_ctx;
   op->A = c.A.view();
@@ -374,6 +383,11 @@ int EVSLStart(void) {
   EVSL_HIP(c.red_part.alloc((size_t)evsl_dev::kRedBlocks * 64 * sizeof(double)));
   EVSL_HIP(c.red_scalars.alloc(64 * sizeof(double)));
   EVSL_HIP(hipHostMalloc((void**)&c.pinned, 4096 * sizeof(double), hipHostMallocDefault));
+  // the allocation-free entry point's scratch on this device: its address
+  // is looked up here once (the entry point retries on its first call if
+  // this fails, and reports the error there)
+  void* scratch = nullptr;
+  if (hipError_t e = evsl_dev::noalloc_scratch(&scratch); e != hipSuccess) (void)hipGetLastError();
   c.started = true;
   StatsReset();
   std::memset(&c.astats, 0, sizeof(c.astats));

@@ -43,8 +43,7 @@ EXPORTS = [
     "evsl_amd_last_error", "evsl_amd_stats_size", "kpmdos_ex", "evsl_amd_bench_filter",
     "ChebLanNr_batch", "evsl_amd_read_coo_mm", "evsl_amd_bench_copy",
     "evsl_amd_spmm_vec", "evsl_amd_spmm_index_bytes", "evsl_amd_cheb_av_cols",
-    "evsl_amd_set_grouped_filter",
-
+    "evsl_amd_cheb_av_cols_noalloc", "evsl_amd_set_grouped_filter",
     "evsl_amd_spmm_streaming", "kpmdos_batch",
     "evsl_amd_server_create", "evsl_amd_server_destroy", "evsl_amd_server_add_matrix",
     "evsl_amd_server_submit_slice", "evsl_amd_server_submit_dos", "evsl_amd_server_step",
@@ -153,6 +152,7 @@ class EVSL:
             "evsl_amd_server_stats": (I, [VP, ip, ip, C.POINTER(C.c_size_t),
                                           C.POINTER(C.c_size_t)]),
             "evsl_amd_cheb_av_cols": (I, [I, pp, dp, dp, I]),
+            "evsl_amd_cheb_av_cols_noalloc": (I, [I, pp, VP, VP, I]),
             "evsl_amd_set_grouped_filter": (I, [I]),
             "ChebLanNr_batch": (I, [I, dp, I, D, dp, pp, ip, ddp, ddp, ddp, VP]),
         }
@@ -317,6 +317,13 @@ class EVSL:
         err = self.lib.evsl_amd_cheb_av_cols(ncols, P, X.ctypes.data_as(c_double_p),
                                              Y.ctypes.data_as(c_double_p), prec)
         return err, Y
+
+    def cheb_av_cols_noalloc(self, pols, Xd, Yd, prec=PREC_FP64):
+        """The allocation-free form of cheb_av_cols: Xd, Yd are device
+        pointers to n x len(pols) column-major blocks.  Returns the error code."""
+        P = (PolParams * len(pols))(*pols)
+        return self.lib.evsl_amd_cheb_av_cols_noalloc(len(pols), P, C.c_void_p(Xd),
+                                                      C.c_void_p(Yd), prec)
 
     def kpmdos_batch(self, mats, intvs, Mdeg, damping, nvec, vec_kind=VEC_RADEMACHER,
                      seeds=None, prec=PREC_FP64):

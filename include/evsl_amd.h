@@ -127,6 +127,22 @@ int ChebLanNr_batch(int nslices, double *intvs, int maxit, double tol, double *v
 int evsl_amd_cheb_av_cols(int ncols, const polparams *pols, const double *X, double *Y,
                           int prec);
 
+/* evsl_amd_cheb_av_cols for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  It allocates and frees no
+ * device memory: its work buffers live in a static 64 MiB device array that
+ * each GPU holds from the library's load until the process ends.  X and Y
+ * (n x ncols, column-major) must be device memory; the device matrix must be
+ * set (SetAMatrix) and, for EVSL_PREC_MIXED, its FP32 copy made by an
+ * earlier mixed-precision call on it.  Columns run in blocks that fit the
+ * scratch; one column needs n x (4 x s + 8) + (deg + 1) x 8 bytes (s = 8 in
+ * FP64, 4 in MIXED), so n up to about 1.6 million rows in FP64 and 2.7
+ * million in MIXED; beyond that EVSL_ERR_ARG.  The result equals
+ * evsl_amd_cheb_av_cols's bit for bit.  Calls that overlap on one GPU share
+ * the scratch and race; each call leaves it holding its own last block. */
+int evsl_amd_cheb_av_cols_noalloc(int ncols, const polparams *pols, const double *X,
+                                  double *Y, int prec);
+
 /* A request loop over device-resident matrices (no counterpart in the
  * reference; its batched slicing and DOS forms served as requests arrive).
  * Matrices registered with evsl_amd_server_add_matrix stay in HBM.  Slice
