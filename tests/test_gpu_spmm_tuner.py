@@ -69,12 +69,16 @@ def test_tuner_candidates_and_argument_errors(ev):
     reset_keys(ev)
 
 
-@pytest.mark.parametrize("case", ["lap3d_i16", "randsym_i32"])
+@pytest.mark.parametrize("case", ["lap3d_i16", "anderson_periodic", "randsym_i32"])
 def test_every_layout_same_bits_chebav(ev, case):
     if case == "lap3d_i16":
         A = M.perturb_diag(M.laplacian(24, 24, 24), 1e-3, seed=2)
         lo, hi = 0.0, 12.001
         ivs = [(0.4, 0.7), (1.5, 1.9), (5.0, 6.0)]
+    elif case == "anderson_periodic":  # wrap-around neighbours far from the row
+        A = M.anderson(18, 16.5, seed=6)
+        lo, hi = M.gershgorin(A)
+        ivs = [(-0.5, 0.5), (2.0, 3.0), (-6.0, -4.0)]
     else:
         A = M.random_symmetric(20000, 8, 32, seed=12)
         lo, hi = M.gershgorin(A)
