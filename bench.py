@@ -260,6 +260,32 @@ def cpu_baseline(A, lo, hi, step, args):
             "cpu": cpu_info()}
 
 
+def dense_bench(args):
+    """The dense FP64 kernels of the solve at the 100^3 workload's sizes:
+    Gram-Schmidt passes (k_gemv_t_partial + k_gemv_n) against k = 128, 300
+    and 600 basis vectors of 10^6 rows (the mean basis length of the mixed
+    solve is ~290), the Ritz GEMM V Z (10^6 x 573 x 266: the last basis
+    times the refinement block) and the Gram product U^T U (10^6 x 266).
+    HBM peak 8 TB/s, FP64 peak 78.6 TFLOP/s (vector and matrix alike)."""
+    from evsl_amd.evsl import EVSL
+    ev = EVSL()
+    assert ev.start() == 0
+    n = 10 ** 6
+    out = []
+    for kind, k, m, name in [(0, 128, 1, "cgs_pass_k128"), (0, 300, 1, "cgs_pass_k300"),
+                             (0, 600, 1, "cgs_pass_k600"), (1, 573, 266, "ritz_gemm_nn"),
+                             (2, 1, 266, "gram_gemm_tn")]:
+        err, ms, by, fl = ev.bench_dense(kind, n, k, m, 10)
+        assert err == 0, ev.last_error()
+        out.append({"kernel": name, "n": n, "k": k, "m": m, "ms": ms,
+                    "GBps": by / (ms * 1e-3) / 1e9, "TFLOPs": fl / (ms * 1e-3) / 1e12,
+                    "frac_hbm": by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    "frac_fp64": fl / (ms * 1e-3) / 1e12 / 78.6})
+        log(f"[bench] {out[-1]}")
+    print(json.dumps(out))
+    ev.finish()
+
+
 def kpm_batch_bench(args):
     """Batched KPM form: args.kb_requests DOS requests (Mdeg 200, 32 Rademacher
     vectors, Jackson damping) over a pool of args.kb_matrices matrices with n
@@ -685,6 +711,11 @@ def main():
     p.add_argument("--filter-only", action="store_true",
                    help="only time the fused filter kernels and a 512 MiB device copy "
                         "(for hardware-counter runs)")
+    p.add_argument("--dense-only", action="store_true",
+                   help="only time the dense FP64 kernels at the 100^3 workload's sizes (a "
+                        "Gram-Schmidt pass against 128/300/600 basis vectors, the Ritz GEMM "
+                        "10^6 x 573 x 266, the Gram product 10^6 x 266): for hardware-counter "
+                        "runs")
     p.add_argument("--kpm-batch", action="store_true",
                    help="time the batched KPM form instead: DOS requests over a pool of "
                         "matrices of uneven size (not the headline line)")
@@ -728,6 +759,9 @@ def main():
         sys.exit(2)
     if args.kpm_batch:
         kpm_batch_bench(args)
+        return
+    if args.dense_only:
+        dense_bench(args)
         return
     if args.serve:
         serve_bench(args)

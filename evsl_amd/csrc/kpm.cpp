@@ -361,6 +361,80 @@ int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, doub
   return 0;
 }
 
+int evsl_amd_bench_dense(int kind, int n, int k, int m, int reps, double* ms_per_rep,
+                         double* bytes, double* flops) {
+  if (int e = ensure_started()) return e;
+  if (kind < 0 || kind > 2 || n <= 0 || k <= 0 || m <= 0 || reps <= 0 || !ms_per_rep ||
+      !bytes || !flops)
+    return EVSL_ERR_ARG;
+  Context& c = ctx();
+  const size_t nn = (size_t)n;
+  DevBuf A, B, C, part, h;
+  // kind 0: A = V (n x k), B = w (n); kind 1: A = V, B = Z (k x m), C = n x m;
+  // kind 2: A = U (n x m), C = m x m
+  const size_t acols = kind == 2 ? (size_t)m : (size_t)k;
+  EVSL_HIP(A.alloc(nn * acols * sizeof(double)));
+  EVSL_HIP(evsl_dev::rm_random(nn * acols, 1, 1, 17, 0, 1, A.as<double>(), c.stream));
+  if (kind == 0) {
+    EVSL_HIP(B.alloc(nn * sizeof(double)));
+    EVSL_HIP(evsl_dev::rm_random(nn, 1, 1, 18, 0, 1, B.as<double>(), c.stream));
+    // V^T w partials (chunks x k), and one norm partial per 256 rows
+    const size_t np = std::max((size_t)evsl_dev::gemv_t_chunks(nn) * (k + 1),
+                               (nn + evsl_dev::kWG - 1) / evsl_dev::kWG);
+    EVSL_HIP(part.alloc(np * sizeof(double)));
+    EVSL_HIP(h.alloc((size_t)(k + 1) * sizeof(double)));
+  } else if (kind == 1) {
+    EVSL_HIP(B.alloc((size_t)k * m * sizeof(double)));
+    EVSL_HIP(evsl_dev::rm_random((size_t)k * m, 1, 1, 19, 0, 1, B.as<double>(), c.stream));
+    EVSL_HIP(C.alloc(nn * m * sizeof(double)));
+  } else {
+    EVSL_HIP(C.alloc((size_t)m * m * sizeof(double)));
+    EVSL_HIP(part.alloc(evsl_dev::gemm_tn_part_elems(nn, m, m) * sizeof(double)));
+  }
+  auto once = [&]() -> hipError_t {
+    if (kind == 0) {
+      hipError_t e = evsl_dev::gemv_t(nn, k, A.as<double>(), nn, B.as<double>(),
+                                      part.as<double>(), h.as<double>(), c.stream);
+      if (e != hipSuccess) return e;
+      // the pass subtracts V h from w: scale h so that w stays bounded
+      e = evsl_dev::scal(k, 1e-30, h.as<double>(), c.stream);
+      if (e != hipSuccess) return e;
+      return evsl_dev::gemv_n_sub_nrm(nn, k, A.as<double>(), nn, h.as<double>(), B.as<double>(),
+                                      part.as<double>(), c.stream);
+    }
+    if (kind == 1)
+      return evsl_dev::gemm_nn(nn, m, k, A.as<double>(), nn, B.as<double>(), k, C.as<double>(),
+                               nn, c.stream);
+    return evsl_dev::gemm_tn(nn, m, m, A.as<double>(), nn, A.as<double>(), nn, C.as<double>(), m,
+                             part.as<double>(), c.stream);
+  };
+  EVSL_HIP(once());  // warm-up
+  hipEvent_t e0, e1;
+  EVSL_HIP(hipEventCreate(&e0));
+  EVSL_HIP(hipEventCreate(&e1));
+  EVSL_HIP(hipEventRecord(e0, c.stream));
+  for (int r = 0; r < reps; ++r) EVSL_HIP(once());
+  EVSL_HIP(hipEventRecord(e1, c.stream));
+  EVSL_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  EVSL_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_per_rep = ms / reps;
+  const double dn = (double)n, dk = (double)k, dm = (double)m;
+  if (kind == 0) {
+    *bytes = 16.0 * dn * dk + 24.0 * dn;  // V twice, w in, out, in
+    *flops = 4.0 * dn * dk;
+  } else if (kind == 1) {
+    *bytes = 8.0 * (dn * dk + dn * dm + dk * dm);
+    *flops = 2.0 * dn * dk * dm;
+  } else {
+    *bytes = 8.0 * (dn * dm + dm * dm);
+    *flops = 2.0 * dn * dm * dm;
+  }
+  return 0;
+}
+
 int evsl_amd_spmm_index_bytes(void) {
   const Context& c = ctx();
   return c.op == 1 ? c.A.idx_bytes() : 0;

@@ -41,7 +41,7 @@ EXPORTS = [
     "evsl_amd_set_refine_passes", "evsl_amd_set_refine_guard", "evsl_amd_set_refine_widen",
     "evsl_amd_get_stats", "evsl_amd_reset_stats",
     "evsl_amd_last_error", "evsl_amd_stats_size", "kpmdos_ex", "evsl_amd_bench_filter",
-    "ChebLanNr_batch", "evsl_amd_read_coo_mm", "evsl_amd_bench_copy",
+    "ChebLanNr_batch", "evsl_amd_read_coo_mm", "evsl_amd_bench_copy", "evsl_amd_bench_dense",
     "evsl_amd_spmm_vec", "evsl_amd_spmm_index_bytes", "evsl_amd_cheb_av_cols",
     "evsl_amd_cheb_av_cols_noalloc", "evsl_amd_set_grouped_filter",
     "evsl_amd_spmm_streaming", "kpmdos_batch",
@@ -132,6 +132,7 @@ class EVSL:
             "evsl_amd_stats_size": (C.c_size_t, []),
             "evsl_amd_bench_filter": (I, [I, I, I, I, D, D, dp, dp]),
             "evsl_amd_bench_copy": (I, [C.c_size_t, I, dp]),
+            "evsl_amd_bench_dense": (I, [I, I, I, I, I, dp, dp, dp]),
             "evsl_amd_spmm_vec": (I, [I, I, I]),
             "evsl_amd_spmm_layout": (I, [I, I, I, C.POINTER(I)]),
             "evsl_amd_set_spmm_layout": (I, [I, I, I, I]),
@@ -422,6 +423,14 @@ class EVSL:
         err = self.lib.evsl_amd_bench_filter(B, prec, mode, nsteps, cc, dd, C.byref(ms),
                                              C.byref(by))
         return err, ms.value, by.value
+
+    def bench_dense(self, kind, n, k, m, reps=10):
+        """(err, ms per repetition, bytes, flops) of a dense kernel (kind 0 a
+        Gram-Schmidt pass, 1 the Ritz GEMM, 2 the Gram product)."""
+        ms, by, fl = C.c_double(), C.c_double(), C.c_double()
+        err = self.lib.evsl_amd_bench_dense(kind, n, k, m, reps, C.byref(ms), C.byref(by),
+                                            C.byref(fl))
+        return err, ms.value, by.value, fl.value
 
     def bench_copy(self, nbytes, reps=20):
         """(err, ms per copy) of a device copy of nbytes; 2*nbytes move."""

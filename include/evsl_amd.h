@@ -199,6 +199,15 @@ int evsl_amd_read_coo_mm(const char *path, int idxin, int idxout, cooMat *A);
 int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, double dd,
                           double *ms_per_step, double *bytes_per_step);
 
+/* Measurement hook: the dense FP64 kernels of the solvers on random data
+ * (n rows), reps times between HIP events: kind 0 one Gram-Schmidt pass
+ * against k basis vectors (V^T w, then w -= V h with the norm partials),
+ * kind 1 the Ritz-vector GEMM V (n x k) Z (k x m), kind 2 the Gram product
+ * U^T U of an n x m block.  Returns the mean time and the algorithmic bytes
+ * and flops of one repetition. */
+int evsl_amd_bench_dense(int kind, int n, int k, int m, int reps, double *ms_per_rep,
+                         double *bytes, double *flops);
+
 /* Lane layout of the fused filter kernel (no counterpart in the reference;
  * its CSR matvec has no layout).  A layout is 10*V + R: V adjacent columns
  * per lane and R rows per lane group.  The ChebAv (mode 1) and residual
