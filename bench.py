@@ -407,7 +407,10 @@ def kb_slice_phase(args, ev, pool, owner, mus):
         sl.append((m, a, bh, min(A.n, max(5 * (target + 2), 300)), int(srng.integers(1 << 30))))
     out, counts = {}, {}
     ev.lib.evsl_amd_set_grouped_filter(0 if args.no_group else 1)
+    ev.lib.evsl_amd_set_mixed_capped_fp64(0 if args.capped_fp32 else 1)
     for name, prec in (("mixed", PREC_MIXED), ("fp64", PREC_FP64)):
+        if name not in args.kb_forms:
+            continue
         srv = Server(ev, 200 << 30, prec)
         t0 = time.perf_counter()
         mids = [srv.add_matrix(A, lo, hi) for A, lo, hi in pool]
@@ -450,15 +453,17 @@ def kb_slice_phase(args, ev, pool, owner, mus):
                      "refine_columns": int(st["n_refine_cols"]),
                      "refine_extra_passes": int(st["n_refine_extra"])}
         log(f"[bench] kpm batch slices {name}: {out[name]}")
+    both = "mixed" in out and "fp64" in out
     diff = [{"matrix": sl[i][0], "family": fams[sl[i][0] % 3], "n": int(pool[sl[i][0]][0].n),
              "mixed": counts["mixed"][i], "fp64": counts["fp64"][i]}
-            for i in range(len(sl)) if counts["mixed"][i] != counts["fp64"][i]]
+            for i in range(len(sl)) if both and counts["mixed"][i] != counts["fp64"][i]]
     return {"slices": len(sl), "eigs_per_slice": list(args.kb_slice_eigs), "tol": 1e-8,
-            "grouped_filter": not args.no_group,
-            "mixed": out["mixed"], "fp64": out["fp64"],
+            "grouped_filter": not args.no_group, "capped_fp32": bool(args.capped_fp32),
+            "per_slice_counts": counts, **out,
             "speedup_eigenpairs_per_s": out["mixed"]["eigenpairs_per_s"] /
-                                        out["fp64"]["eigenpairs_per_s"],
-            "counts_equal": counts["mixed"] == counts["fp64"], "differing_slices": diff}
+                                        out["fp64"]["eigenpairs_per_s"] if both else None,
+            "counts_equal": counts["mixed"] == counts["fp64"] if both else None,
+            "differing_slices": diff}
 
 
 def kpm_count(mu, n, lo, hi, a, b):
@@ -711,6 +716,9 @@ def main():
     p.add_argument("--filter-only", action="store_true",
                    help="only time the fused filter kernels and a 512 MiB device copy "
                         "(for hardware-counter runs)")
+    p.add_argument("--filter-layout", type=int, default=0,
+                   help="--filter-only: the filter step's lane layout (10 V + R; 0: the fixed "
+                        "policy)")
     p.add_argument("--dense-only", action="store_true",
                    help="only time the dense FP64 kernels at the 100^3 workload's sizes (a "
                         "Gram-Schmidt pass against 128/300/600 basis vectors, the Ritz GEMM "
@@ -727,6 +735,10 @@ def main():
                    help="second phase: slices drawn from the pool (0: none)")
     p.add_argument("--kb-slice-eigs", type=int, nargs=2, default=[20, 60],
                    help="eigenvalues per slice of the second phase (range)")
+    p.add_argument("--kb-forms", nargs="+", default=["mixed", "fp64"],
+                   choices=["mixed", "fp64"], help="second phase: servers to run, in turn")
+    p.add_argument("--capped-fp32", action="store_true",
+                   help="mixed form: run capped filters (deg = max_deg - 1) in FP32 too")
     p.add_argument("--c3", action="store_true",
                    help="run the precision-form workload instead (3-D Anderson 80^3, 16 slices "
                         "of [-0.05, 0.05]) in --form mixed or batch64, dumping eigenvalues "
@@ -781,14 +793,22 @@ def main():
     cc, dd = 0.5 * (lo + hi), 0.5 * (hi - lo)
     if args.filter_only:
         # the dominant kernels at their solver widths, then a 512 MiB copy of
-        # known size that calibrates the hardware byte counters.  The fixed
-        # layout policy runs: under counter collection kernel times are not
-        # representative, so the layout tuner would choose on noise; the
-        # tuned layouts of these keys on this matrix differ from the table
-        # only in rows per lane group (the same kernel instance)
+        # known size that calibrates the hardware byte counters.  Under
+        # counter collection kernel times are not representative, so the
+        # layout tuner would choose on noise: the layouts are fixed, the
+        # filter's to --filter-layout (the one the timed run's tuner chose
+        # on this matrix), the others to the fixed policy
         ev.set_spmm_tuning(False)
         prec = PREC_MIXED if args.form == "mixed" else PREC_FP64
         B = args.nslices if args.form != "plain" else 1
+        if args.filter_layout:
+            ev.set_spmm_tuning(True)
+            assert ev.set_spmm_layout(B, prec, 1, args.filter_layout) == 0, ev.last_error()
+            for b in (2, 4, 8, 16, 32):
+                for pr in (PREC_FP64, PREC_MIXED):
+                    for md in (1, 3):
+                        if (b, pr, md) != (B, prec, 1):
+                            assert ev.set_spmm_layout(b, pr, md, 0) == 0
         out = {}
         for tag, (b, pr, mode) in {"filter": (B, prec, 1), "refine": (32, prec, 3),
                                    "kpm": (32, PREC_FP64, 2)}.items():

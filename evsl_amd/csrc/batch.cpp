@@ -567,12 +567,16 @@ int refine_extract(int n, int kdim, const double* dT, const double* eT, const do
   } else {
     // Refinement passes: filter the block with p(A) (FP32 residual
     // recurrence), re-orthonormalize (Cholesky QR twice) and extract Ritz
-    // pairs with A in FP64.  Beyond the requested passes, at most
-    // kMaxExtraPasses more run while a Ritz value inside [a, b] is close to
-    // converged (tol <= residual < kRetryFactor * tol): a pass damps the
-    // components outside the guard set by at least the guard ratio.
-    constexpr int kMaxExtraPasses = 2;
+    // pairs with A in FP64.  Beyond the requested passes, more run while a
+    // Ritz value inside [a, b] is close to converged (tol <= residual <
+    // kRetryFactor * tol): a pass damps the components outside the guard set
+    // by the guard ratio, so such a pair needs a few.  They go on while the
+    // largest near-converged residual falls by kProgress per pass, at most
+    // kMaxExtraPasses; then the slice counts as stalled.
+    constexpr int kMaxExtraPasses = 8;
     constexpr double kRetryFactor = 1e3;
+    constexpr double kProgress = 0.9;
+    double last_near = 0.0;
     // the first pass's polynomial: the interval widened by refine_widen
     // slice widths on each side, same damping and thresholds (a lower
     // degree; the FP32-filtered basis carries rounding noise spread over
@@ -671,15 +675,22 @@ int refine_extract(int n, int kdim, const double* dT, const double* eT, const do
       if (int err = measure()) return err;
       if (pass + 1 < min_pass) continue;
       int near = 0;
+      double near_max = 0.0;
       for (int j = 0; j < m; ++j) {
         const double r = std::sqrt(std::max(hr2[j], 0.0));
-        if (ht[j] >= aa - kEpsM && ht[j] <= bb + kEpsM && !(r < tol) && r < kRetryFactor * tol)
+        if (ht[j] >= aa - kEpsM && ht[j] <= bb + kEpsM && !(r < tol) &&
+            r < kRetryFactor * tol) {
           ++near;
+          near_max = std::max(near_max, r);
+        }
       }
       if (fstats)
-        std::fprintf(fstats, "  refinement pass %d: %d near-converged in [a,b]\n", pass + 1, near);
+        std::fprintf(fstats, "  refinement pass %d: %d near-converged in [a,b] (max %.3e)\n",
+                     pass + 1, near, near_max);
       if (near == 0) break;
-      if (pass + 1 >= min_pass + kMaxExtraPasses) {
+      const bool progress = pass + 1 == min_pass || near_max < kProgress * last_near;
+      last_near = near_max;
+      if (pass + 1 >= min_pass + kMaxExtraPasses || !progress) {
         // the passes no longer close the gap to tol (a filter too weak to
         // separate the guard set): the caller may redo the slice in FP64
         if (stalled) *stalled = true;
@@ -874,7 +885,8 @@ int LanczosGroup::add(const double* intv, int maxit, double tol, const double* v
   // (deg = max_deg - 1, SRC/chebpoly.c:363-490) separates its interval
   // weakly, its slice tends to end at maxit, and there the FP32-filtered
   // basis leaves pairs just above tol and the slice would be redone in FP64
-  const bool capped = pol->max_deg > 0 && pol->deg >= pol->max_deg - 1;
+  const bool capped =
+      c.mixed_capped_fp64 && pol->max_deg > 0 && pol->deg >= pol->max_deg - 1;
   const bool f32 =
       g.mixed && !capped && pol->deg >= c.mixed_min_deg && pol->deg <= c.mixed_max_deg;
   if (g.mixed && !f32) ++c.astats.n_fp64_slices;
@@ -929,6 +941,8 @@ int LanczosGroup::step_cols(std::vector<FilterCol>& cols) {
     }
   return 0;
 }
+
+int LanczosGroup::filter_cols(const std::vector<FilterCol>& cols) { return p_->filter(cols); }
 
 int LanczosGroup::step() {
   Impl& g = *p_;

@@ -97,6 +97,7 @@ struct Context {
   double refine_widen = 2.0;
   int mixed_min_deg = 32;    // mixed form: filters of degree outside
   int mixed_max_deg = 20000; // [mixed_min_deg, mixed_max_deg] run in FP64
+  bool mixed_capped_fp64 = true;  // mixed form: capped filters run in FP64
   bool spmm_tuning = true;   // filter launches go through the matrix's layout tuner
   // the request loop filters the slices of all its matrices in grouped
   // launches (else one matrix at a time)
@@ -236,6 +237,10 @@ class LanczosGroup {
   int start_done();                              // normalise them, activate
   int step_cols(std::vector<FilterCol>& cols);   // the active slices' vectors
   int step_done();  // recurrence, re-orthogonalisation, tests, results
+  // filters columns of this group's matrix (bound) with its own blocked
+  // filter: tuned lane layouts, columns of lower degree dropped as the series
+  // passes their degree
+  int filter_cols(const std::vector<FilterCol>& cols);
 
  private:
   struct Impl;

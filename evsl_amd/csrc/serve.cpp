@@ -206,12 +206,24 @@ int admit(evslAmdServer* s) {
   return 0;
 }
 
-// One filter pass of the slices of every listed matrix in grouped launches:
-// the start vectors of slices added since the last step (start), or the
-// current Lanczos vectors of the active slices; then, for a start pass, the
+// A matrix whose filter launch moves more than this is filtered on its own
+// (the launch fills the device by itself; its own filter keeps its tuned
+// lane layout and drops columns as their degrees end); smaller ones, whose
+// launches are latency-bound, share grouped launches.
+constexpr double kGroupLaunchBytes = 32e6;
+
+// One filter pass of the slices of every listed matrix: the start vectors
+// of slices added since the last step (start), or the current Lanczos
+// vectors of the active slices -- the small matrices' columns in grouped
+// launches, each large one with its own filter; then, for a start pass, the
 // new slices are normalised and join.
+
 int grouped_pass(evslAmdServer* s, const std::vector<int>& live, bool start) {
   Context& c = ctx();
+  if (!s->ev[0])
+    for (auto& e : s->ev) EVSL_HIP(hipEventCreate(&e));
+  EVSL_HIP(hipEventRecord(s->ev[0], c.stream));
+  bool any = false;
   std::vector<GroupedCols> parts;
   for (int mi : live) {
     SrvMatrix& m = *s->mats[mi];
@@ -223,18 +235,24 @@ int grouped_pass(evslAmdServer* s, const std::vector<int>& live, bool start) {
       for (const auto& col : cols)
         if (col.f32 == (pr == 1)) gc.cols.push_back(col);
       if (gc.cols.empty()) continue;
+      any = true;
       // (the operator's view carries this matrix's sizes; inside the
       // binding m.M holds the context's previous matrix)
       const polparams* p0 = gc.cols[0].pol;
       if (int e = make_filter_op(p0->cc, p0->dd, pr == 1, &gc.op)) return e;
-      parts.push_back(std::move(gc));
+      const double vs = pr == 1 ? 4.0 : 8.0;
+      const double launch = (double)gc.op.A.nnz * (vs + (gc.op.A.jd ? 2 : 4)) +
+                            (double)gc.op.A.n * gc.cols.size() * (3.0 * vs + 16.0 / 3.0);
+      if (launch > kGroupLaunchBytes) {
+        if (int e = m.grp->filter_cols(gc.cols)) return e;
+      } else {
+        parts.push_back(std::move(gc));
+      }
     }
   }
-  if (!parts.empty()) {
-    if (!s->ev[0])
-      for (auto& e : s->ev) EVSL_HIP(hipEventCreate(&e));
-    EVSL_HIP(hipEventRecord(s->ev[0], c.stream));
-    if (int e = grouped_filter(parts, s->gw)) return e;
+  if (any) {
+    if (!parts.empty())
+      if (int e = grouped_filter(parts, s->gw)) return e;
     EVSL_HIP(hipEventRecord(s->ev[1], c.stream));
     float ms = 0.f;
     EVSL_HIP(hipEventSynchronize(s->ev[1]));

@@ -44,6 +44,7 @@ EXPORTS = [
     "ChebLanNr_batch", "evsl_amd_read_coo_mm", "evsl_amd_bench_copy", "evsl_amd_bench_dense",
     "evsl_amd_spmm_vec", "evsl_amd_spmm_index_bytes", "evsl_amd_cheb_av_cols",
     "evsl_amd_cheb_av_cols_noalloc", "evsl_amd_set_grouped_filter",
+    "evsl_amd_set_mixed_capped_fp64",
     "evsl_amd_spmm_streaming", "kpmdos_batch",
     "evsl_amd_server_create", "evsl_amd_server_destroy", "evsl_amd_server_add_matrix",
     "evsl_amd_server_submit_slice", "evsl_amd_server_submit_dos", "evsl_amd_server_step",
@@ -155,6 +156,7 @@ class EVSL:
             "evsl_amd_cheb_av_cols": (I, [I, pp, dp, dp, I]),
             "evsl_amd_cheb_av_cols_noalloc": (I, [I, pp, VP, VP, I]),
             "evsl_amd_set_grouped_filter": (I, [I]),
+            "evsl_amd_set_mixed_capped_fp64": (I, [I]),
             "ChebLanNr_batch": (I, [I, dp, I, D, dp, pp, ip, ddp, ddp, ddp, VP]),
         }
         for name, (res, args) in sig.items():

@@ -52,7 +52,8 @@ extern "C" {
 int evsl_amd_set_filter_precision(int prec);
 int evsl_amd_get_filter_precision(void);
 /* number of filtered Rayleigh-Ritz refinement passes in MIXED mode (>= 1);
- * up to two more run while a Ritz value in [a, b] is near convergence */
+ * more run while a Ritz value in [a, b] is near convergence (residual below
+ * 1000 tol) and the largest such residual falls by 10% per pass, at most 8 */
 int evsl_amd_set_refine_passes(int passes);
 /* guard threshold of the refinement: Ritz values of p(A) down to
  * guard * bar enter the FP64 Rayleigh-Ritz space (0 < guard <= 1,
@@ -75,6 +76,9 @@ int evsl_amd_set_refine_widen(double widen);
  * mixed slice whose refinement stalls above tol after its extra passes is
  * redone from its start vector in FP64 (evslAmdStats.n_fp64_reruns). */
 int evsl_amd_set_mixed_degree_range(int min_deg, int max_deg);
+/* MIXED mode: whether a capped filter (deg = max_deg - 1) runs in FP64 (1,
+ * the default) or, like the others, in FP32 (0) */
+int evsl_amd_set_mixed_capped_fp64(int on);
 
 int evsl_amd_get_stats(evslAmdStats *s);
 void evsl_amd_reset_stats(void);
@@ -153,10 +157,11 @@ int evsl_amd_cheb_av_cols_noalloc(int ncols, const polparams *pols, const double
  * queued by submit; each evsl_amd_server_step admits queued requests in
  * arrival order while their projected memory (n x (maxit+2) x 8 bytes for a
  * slice) fits mem_budget, advances every running slice by one filtered
- * Lanczos step -- the filter passes of all matrices' slices in grouped
- * launches (one launch per degree covers the row blocks of every matrix whose
- * filters have not reached it; evsl_amd_set_grouped_filter(0) filters one
- * matrix at a time instead) -- and every running DOS request by up to 64
+ * Lanczos step -- the filter passes of the small matrices' slices (launches
+ * of at most 32 MB) in grouped launches (one launch per degree covers the
+ * row blocks of every such matrix whose filters have not reached it), each
+ * larger matrix with its own filter; evsl_amd_set_grouped_filter(0) filters
+ * every matrix on its own -- and every running DOS request by up to 64
  * Chebyshev degrees in grouped launches.  A request's result does not depend
  * on the other requests sharing its steps (bit for bit).  step returns the
  * number of requests not yet done (0 when idle) or an error.  status returns
