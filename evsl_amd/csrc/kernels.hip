@@ -264,15 +264,22 @@ __global__ void k_f32_to_f64(size_t n, const float* __restrict__ x, double* __re
 constexpr int kGtCols = 8;     // columns per workgroup in V^T w
 constexpr int kGtRows = 4096;  // rows per chunk in V^T w
 
-// part[c * nch + chunk] = sum over the chunk's rows of V(r, c) * w(r)
+// part[c * nch + chunk] = sum over the chunk's rows of V(r, c) * w(r).
+// One workgroup per (chunk, 8-column group) on a 1-D grid of nch * ncg: the
+// XCD remap gives each XCD a contiguous range of that index, and the column
+// group varies fastest, so the workgroups that read one chunk of w run on
+// the same XCD together and w is fetched into its L2 once, not once per
+// column group.
 __global__ __launch_bounds__(kWG) void k_gemv_t_partial(size_t n, int k,
                                                         const double* __restrict__ V,
                                                         size_t ld,
                                                         const double* __restrict__ w,
-                                                        double* __restrict__ part, int nch) {
+                                                        double* __restrict__ part, int nch,
+                                                        int ncg) {
   __shared__ double sh[4][kGtCols];
-  const int ch = blockIdx.x;
-  const int c0 = blockIdx.y * kGtCols;
+  const int lb = xcd_remap(blockIdx.x, nch * ncg);
+  const int ch = lb / ncg;
+  const int c0 = (lb % ncg) * kGtCols;
   const int nc = min(kGtCols, k - c0);
   double acc[kGtCols];
 #pragma unroll
@@ -1116,8 +1123,9 @@ hipError_t gemv_t(size_t n, int k, const double* V, size_t ld, const double* w,
                   double* part, double* h, hipStream_t st) {
   if (k <= 0) return hipSuccess;
   const int nch = gemv_t_chunks(n);
-  dim3 grid(nch, (k + kGtCols - 1) / kGtCols);
-  hipLaunchKernelGGL(k_gemv_t_partial, grid, dim3(kWG), 0, st, n, k, V, ld, w, part, nch);
+  const int ncg = (k + kGtCols - 1) / kGtCols;
+  hipLaunchKernelGGL(k_gemv_t_partial, dim3((unsigned)nch * ncg), dim3(kWG), 0, st, n, k, V,
+                     ld, w, part, nch, ncg);
   hipLaunchKernelGGL(k_gemv_t_reduce, dim3(k), dim3(64), 0, st, part, nch, h);
   return hipGetLastError();
 }

@@ -105,6 +105,31 @@ __device__ __forceinline__ void cheb_spmm_block(
   // sum of a long series would otherwise carry the rounding of every one
   // of its additions, noise that the filter does not damp (each term is
   // widened to FP64 and added there).
+  // With V > 1, kCheb's per-column coefficients live in LDS (s_cf: s, s1,
+  // s2, mu0 for the B columns), read by the epilogue: held in registers
+  // over the whole row walk they take 4 V FP64 registers per lane, which
+  // set the occupancy of the V = 2 and V = 4 layouts (76 -> 60 and 126 -> 94
+  // VGPRs at 8 FP32 columns); at V = 1 the 8 registers cost no occupancy
+  // and the LDS reads in every row's epilogue would (the FP64 one-column
+  // step ran 14% slower).  kRes's are the same for every column (scalars).
+  // Thread c < B loads column c's four here and publishes them to s_cf
+  // just before the block's first barrier, so that the loads overlap the
+  // staging loads instead of waiting ahead of them.
+  constexpr bool kCfLds = MODE == kCheb && V > 1;
+  __shared__ double s_cf[kCfLds ? 4 * B : 1];
+  double cfv[4] = {0.0, 0.0, 0.0, 0.0};
+  if (kCfLds && tid < B) {
+    cfv[0] = scol[tid];
+    if (yflush >= 2) cfv[1] = scol[tid - B];
+    if (yflush >= 3) cfv[2] = scol[tid - 2 * B];
+    if (first) cfv[3] = mu0col[tid];
+  }
+  auto publish_cf = [&]() {
+    if (kCfLds && tid < B) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) s_cf[i * B + tid] = cfv[i];
+    }
+  };
   double s[V], s1[V], s2[V], mu0[V];
   VT g[V];
 #pragma unroll
@@ -112,7 +137,7 @@ __device__ __forceinline__ void cheb_spmm_block(
     s[j] = s1[j] = s2[j] = 0;
     mu0[j] = 0;
     g[j] = 0;
-    if (MODE == kCheb) {
+    if (MODE == kCheb && !kCfLds) {
       s[j] = scol[c0 + j];
       if (yflush >= 2) s1[j] = scol[c0 + j - B];
       if (yflush >= 3) s2[j] = scol[c0 + j - 2 * B];
@@ -195,10 +220,17 @@ __device__ __forceinline__ void cheb_spmm_block(
         VecY yn;
 #pragma unroll
         for (int j = 0; j < V; ++j) {
-          double yv = first ? mu0[j] * (double)o.xr.v[j] : o.yo.v[j];
-          if (yflush >= 3) yv = yv + s2[j] * (double)o.km1.v[j];
-          if (yflush >= 2) yv = yv + s1[j] * (double)o.xr.v[j];
-          yn.v[j] = yv + s[j] * (double)w.v[j];
+          double cs = s[j], cs1 = s1[j], cs2 = s2[j], cm = mu0[j];
+          if (kCfLds) {
+            cs = s_cf[c0 + j];
+            cs1 = s_cf[B + c0 + j];
+            cs2 = s_cf[2 * B + c0 + j];
+            cm = s_cf[3 * B + c0 + j];
+          }
+          double yv = first ? cm * (double)o.xr.v[j] : o.yo.v[j];
+          if (yflush >= 3) yv = yv + cs2 * (double)o.km1.v[j];
+          if (yflush >= 2) yv = yv + cs1 * (double)o.xr.v[j];
+          yn.v[j] = yv + cs * (double)w.v[j];
         }
         st_last(y, i, yn);
       }
@@ -229,15 +261,49 @@ __device__ __forceinline__ void cheb_spmm_block(
     // R = 1 launches pass no buffer and keep their LDS footprint
     const bool ia_lds = LDS && s_ia != nullptr && r1 - r0 <= kRowsCap;
     if (LDS) {
-      for (int p = tid; p < nz; p += kWG) {
-        s_ja[p] = jx[p0 + p];
-        s_a[p] = av[p0 + p];
+      // a block holds at most kNnzCap = kStageIt * kWG nonzeros and
+      // kRowsCap + 1 <= 2 kWG row pointers: each thread issues all its
+      // staging loads before it writes any of them to LDS, so that they are
+      // in flight together (one memory round trip per block, not one per
+      // kWG nonzeros)
+      constexpr int kStageIt = kNnzCap / kWG;
+      static_assert(kNnzCap % kWG == 0 && kRowsCap + 1 <= 2 * kWG, "staging shape");
+      IT jv[kStageIt];
+      VT avv[kStageIt];
+      int iv[2] = {0, 0};
+#pragma unroll
+      for (int u = 0; u < kStageIt; ++u) {
+        const int p = tid + u * kWG;
+        if (p < nz) {
+          jv[u] = jx[p0 + p];
+          avv[u] = av[p0 + p];
+        }
       }
-      if (ia_lds)
-        for (int p = tid; p <= r1 - r0; p += kWG) s_ia[p] = ia[r0 + p] - p0;
+      if (ia_lds) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+          if (tid + u * kWG <= r1 - r0) iv[u] = ia[r0 + tid + u * kWG];
+      }
+#pragma unroll
+      for (int u = 0; u < kStageIt; ++u) {
+        const int p = tid + u * kWG;
+        if (p < nz) {
+          s_ja[p] = jv[u];
+          s_a[p] = avv[u];
+        }
+      }
+      if (ia_lds) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+          if (tid + u * kWG <= r1 - r0) s_ia[tid + u * kWG] = iv[u] - p0;
+      }
+      publish_cf();
       __syncthreads();
       jsrc = s_ja;
       asrc = s_a;
+    } else if (kCfLds) {
+      publish_cf();
+      __syncthreads();
     }
     for (int r = rf; act && r < r1; r += NG) {
       int q0 = qf0, q1 = qf1;
@@ -304,6 +370,7 @@ __device__ __forceinline__ void cheb_spmm_block(
 #pragma unroll
       for (int j = 0; j < V; ++j) s_str[st * B + c0 + j] = (double)acc[j];
     }
+    publish_cf();
     __syncthreads();
     if (act && grp == 0) {
       VT tot[V];
