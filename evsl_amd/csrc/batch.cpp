@@ -131,8 +131,14 @@ evsl_dev::ColPtrs col_ptrs(const double* const* p, int cnt) {
 // decreasing degree, the series runs at width b up to the smallest degree,
 // whose columns are then written out and dropped; the live recurrence
 // vectors T_k, T_{k-1} and the partial sums Y are narrowed to the remaining
-// columns, and so on.  Each column sees the same arithmetic as in one
-// full-width run (per-column sums do not depend on the block width).
+// columns, and so on.  An odd remainder of 3, 5 or 7 columns keeps one
+// finished column as a passenger (zero coefficients, its output already
+// written): odd widths have only the one-column-per-lane layout, and at
+// 7 columns it ran 45.6 us a step against 30.5 us for 8 (5: 35.4 against
+// 33.2 for 6; 3: 24.9 against 21.8 for 4; 1 stays 1, at 13.9 against 18.2
+// for 2; FP32 on the 100^3 Laplacian).  Each column sees the same
+// arithmetic as in one full-width run (per-column sums do not depend on the
+// block width).
 int filter_columns_phased(int b, const double* const* xin, double* const* yout,
                           const polparams* const* pols, bool f32, BlockWork& W) {
   Context& c = ctx();
@@ -151,26 +157,31 @@ int filter_columns_phased(int b, const double* const* xin, double* const* yout,
     out[i] = yout[ord[i]];
     key[i] = pols[ord[i]];
   }
-  // phase p runs degrees up to the smallest degree of its active columns;
-  // its table holds coefficient rows 0..kend for those columns
+  // phase p runs degrees up to the smallest degree of its live columns, at
+  // width `active` (the live columns and at most one passenger); its table
+  // holds coefficient rows 0..kend for those columns, zero past a column's
+  // degree.  The next phase keeps `keep` live columns at width `next`.
   auto phases = [&](auto&& visit) {
-    int active = b, k0 = 1;
-    while (active > 0) {
-      const int kend = key[active - 1]->deg;
-      int keep = active;
+    int live = b, active = b, k0 = 1;
+    while (live > 0) {
+      const int kend = key[live - 1]->deg;
+      int keep = live;
       while (keep > 0 && key[keep - 1]->deg == kend) --keep;
-      visit(active, k0, kend, keep);
-      active = keep;
+      const int next = (keep >= 3 && (keep & 1) && keep + 1 <= active) ? keep + 1 : keep;
+      visit(active, live, k0, kend, keep, next);
+      live = keep;
+      active = next;
       k0 = kend + 1;
     }
   };
   if (int e = W.coef_table(key, 0, [&](std::vector<double>& tab, std::vector<size_t>& off) {
-        phases([&](int active, int, int kend, int) {
+        phases([&](int active, int, int, int kend, int, int) {
           off.push_back(tab.size());
           tab.resize(tab.size() + (size_t)(kend + 1) * active, 0.0);
           double* t = tab.data() + off.back();
           for (int i = 0; i < active; ++i)
-            for (int d = 0; d <= kend; ++d) t[(size_t)d * active + i] = key[i]->mu[d];
+            for (int d = 0; d <= kend && d <= key[i]->deg; ++d)
+              t[(size_t)d * active + i] = key[i]->mu[d];
         });
       }))
     return e;
@@ -182,27 +193,27 @@ int filter_columns_phased(int b, const double* const* xin, double* const* yout,
   void* Yspare = W.Y2.p;
   const size_t vs = f32 ? 4 : 8;
   int phase = 0, err = 0;
-  phases([&](int active, int k0, int kend, int keep) {
+  phases([&](int active, int live, int k0, int kend, int keep, int next) {
     if (err) return;
     const double* mus_d = W.mus.as<double>() + W.mus_off[phase++];
     if ((err = block_filter_seg(active, op, mus_d, k0, kend, W.X0.p, slot, Y))) return;
-    hipError_t he = evsl_dev::unpack_range(n, active, keep, active, Y, false,
-                                           col_ptrs(out.data() + keep, active - keep), c.stream);
-    if (he == hipSuccess && keep > 0) {
+    hipError_t he = evsl_dev::unpack_range(n, active, keep, live, Y, false,
+                                           col_ptrs(out.data() + keep, live - keep), c.stream);
+    if (he == hipSuccess && next > 0 && next < active) {
       // T_kend sits in slot (kend-1)%3 and T_{kend-1} in slot (kend-2)%3
       // (kend >= 3); narrow them into the free slots and swap the pointers
       const int sa = (kend - 1) % 3, sb = (kend - 2) % 3, sf = kend % 3;
       void* pa = slot[sa];
       void* pb = slot[sb];
       void* pf = slot[sf];
-      he = evsl_dev::narrow_cols(n, active, keep, pa, pf, f32, c.stream);
-      if (he == hipSuccess) he = evsl_dev::narrow_cols(n, active, keep, pb, pa, f32, c.stream);
+      he = evsl_dev::narrow_cols(n, active, next, pa, pf, f32, c.stream);
+      if (he == hipSuccess) he = evsl_dev::narrow_cols(n, active, next, pb, pa, f32, c.stream);
       slot[sa] = pf;
       slot[sb] = pa;
       slot[sf] = pb;
-      if (he == hipSuccess) he = evsl_dev::narrow_cols(n, active, keep, Y, Yspare, false, c.stream);
+      if (he == hipSuccess) he = evsl_dev::narrow_cols(n, active, next, Y, Yspare, false, c.stream);
       std::swap(Y, Yspare);
-      c.astats.filter_bytes += (2.0 * vs + 8.0) * (double)n * (active + keep);
+      c.astats.filter_bytes += (2.0 * vs + 8.0) * (double)n * (active + next);
     }
     if (he != hipSuccess) {
       set_error("filter_columns_phased: unpack/narrow", he);
