@@ -60,7 +60,7 @@ int block_filter_seg(int B, const FilterOp& op, const double* mus_d, int k0, int
     EVSL_HIP(evsl_dev::cheb_spmm(op.A, B, op.f32, evsl_dev::kCheb, xk, xkm1, xkp1, Y, nullptr,
                                  mus_d + (size_t)k * B, mus_d, k == 1 ? op.t1 : op.t2, op.cc,
                                  k == 1, yf, nullptr, c.stream));
-    c.astats.filter_bytes += (double)c.A.nnz * (vs + c.A.idx_bytes()) + 4.0 * (c.n + 1) +
+    c.astats.filter_bytes += c.A.mat_bytes((int)vs, B) + 4.0 * (c.n + 1) +
                              evsl_dev::cheb_vec_bytes(yf, k == 1, (int)vs) * B * c.n;
   }
   c.astats.n_filter_steps += k1 - k0 + 1;
@@ -492,7 +492,7 @@ int residual_filter(int n, int m, double* U, double* W, const polparams* pol, Bl
                                        gch + (size_t)k * B, mp_d.as<double>() + 2 * k, op.t2,
                                        op.cc, k == 1, yf, nullptr, c.stream));
           // the kCheb streams plus the residual block R
-          c.astats.refine_bytes += (double)c.A.nnz * (4 + c.A.idx_bytes()) + 4.0 * (n + 1) +
+          c.astats.refine_bytes += c.A.mat_bytes(4, B) + 4.0 * (n + 1) +
                                    (evsl_dev::cheb_vec_bytes(yf, k == 1, 4) + 4.0) * B * n;
         }
         EVSL_HIP(evsl_dev::unpack_cols(n, B, Yb, false, col_ptrs(out.data(), B), c.stream));

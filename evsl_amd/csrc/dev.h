@@ -28,6 +28,9 @@ inline int rows_cap_for(int B) { return kWG / B; }
 
 struct SpmmTuner;
 
+constexpr int kPatCodes = 255;
+constexpr unsigned char kPatDiag = 255;
+
 struct Csr {
   int n = 0, nnz = 0;
   const int* ia = nullptr;
@@ -37,12 +40,39 @@ struct Csr {
   const short* jd = nullptr;
   const double* a64 = nullptr;
   const float* a32 = nullptr;
+  // Pattern codes, when the matrix's off-diagonal entries take at most
+  // kPatCodes distinct (column offset, value) pairs (stencil and lattice
+  // operators): one byte per nonzero, pc[k] < kPatCodes naming the pair
+  // (poff[pc[k]], pval[pc[k]]) and pc[k] == kPatDiag the row's diagonal,
+  // whose value is dg[row].  The filter kernels then stream one byte per
+  // nonzero instead of an index and a value, with the same products in the
+  // same order.  pval32/dg32 always go with a32 (both shifted or neither).
+  // pdpos[row]: the diagonal's position within its row (kPatDiag: none).
+  const unsigned char* pc = nullptr;
+  const unsigned char* pdpos = nullptr;
+  const int* poff = nullptr;
+  bool pshort = false;  // every poff fits int16
+  const double* pval64 = nullptr;
+  const double* dg64 = nullptr;
+  const float* pval32 = nullptr;
+  const float* dg32 = nullptr;
   // per block width: nblk+1 pairs (first row of the block, its row pointer)
   const int* rb[kNumWidths] = {};
   int nblk[kNumWidths] = {};
   // the matrix's layout tuner (host object; null: the fixed layout policy)
   SpmmTuner* tune = nullptr;
 };
+
+// Launches whose vector rows are at most this many bytes (B * value size)
+// read pattern codes; wider ones read the index and value arrays.  Decoding
+// costs a table lookup per nonzero ahead of the gathers: on the 100^3
+// Laplacian it paid at 1 and 2 FP32 columns and 1 FP64 column (11.2 against
+// 13.7 us, 16.2 against 19.1, 16.2 against 19.3) and cost or tied at 4-32
+// columns, where the vectors carry most of a launch's bytes.
+constexpr int kPatMaxRowBytes = 8;
+inline bool pattern_launch(const Csr& A, int B, int vsize) {
+  return A.pc != nullptr && B * vsize <= kPatMaxRowBytes;
+}
 
 enum Mode { kPlain = 0, kCheb = 1, kKpm = 2, kRes = 3 };
 
@@ -279,6 +309,10 @@ hipError_t fill_u32(size_t n, unsigned v, unsigned* x, hipStream_t st);
 // FP32 copy of (A - cc I) * scale (diagonal entries shifted); rows must
 // hold a stored diagonal entry
 hipError_t shift_scale_f32(const Csr& A, double cc, double scale, float* out, hipStream_t st);
+// out[r] = (float)((d[r] - cc) * scale): the diagonal of the shifted FP32
+// copy, with shift_scale_f32's arithmetic
+hipError_t shift_diag_f32(int n, const double* d, double cc, double scale, float* out,
+                          hipStream_t st);
 // Z (k x k, column-major) = identity
 hipError_t set_identity(int k, double* Z, hipStream_t st);
 // precision conversions

@@ -61,8 +61,30 @@ struct DevMatrix {
   int nblk[evsl_dev::kNumWidths] = {};
   evsl_dev::SpmmTuner* tune = nullptr;  // the filter kernel's layout tuner
   bool owns_csr = false;  // ia/ja/a64 allocated here (SetAMatrix)
+  // pattern codes (evsl_dev::Csr::pc): one byte per nonzero, tables of at
+  // most kPatCodes (column offset, value) pairs, the diagonal per row; the
+  // FP32 tables go with a32 (unshifted) and a32s (shifted)
+  unsigned char* pc = nullptr;
+  unsigned char* pdpos = nullptr;  // the diagonal's position in its row
+  int* poff = nullptr;
+  bool pshort = false;
+  int npat = 0;                  // table entries in use
+  std::vector<double> pval_h;    // host copy of the FP64 table values
+  double* pval64 = nullptr;
+  double* dg64 = nullptr;
+  float* pval32 = nullptr;
+  float* dg32 = nullptr;
+  float* pval32s = nullptr;
+  float* dg32s = nullptr;
   // bytes per nonzero of the column index the filter kernels read
-  int idx_bytes() const { return jd ? 2 : 4; }
+  int idx_bytes() const { return pc ? 1 : (jd ? 2 : 4); }
+  // matrix bytes one filter launch of B columns streams with values of vs
+  // bytes: the codes, the diagonal and its position, or the indices and the
+  // values (evsl_dev::pattern_launch)
+  double mat_bytes(int vs, int B) const {
+    return (pc && B * vs <= evsl_dev::kPatMaxRowBytes) ? (double)nnz + (double)n * (vs + 1)
+                                                       : (double)nnz * (vs + (jd ? 2 : 4));
+  }
   evsl_dev::Csr view() const;
   void release();
 };
@@ -99,6 +121,7 @@ struct Context {
   int mixed_max_deg = 20000; // [mixed_min_deg, mixed_max_deg] run in FP64
   bool mixed_capped_fp64 = true;  // mixed form: capped filters run in FP64
   bool spmm_tuning = true;   // filter launches go through the matrix's layout tuner
+  bool pattern_codes = true; // matrices built from now on use pattern codes when they can
   // the request loop filters the slices of all its matrices in grouped
   // launches (else one matrix at a time)
   bool grouped_filter = true;

@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kWG) void k_kpm_group(const KpmGroupEntry* __restri
   const VT* xk = k == 0 ? x0 : ring + (size_t)((k - 1) % 3) * nb;
   const VT* xkm1 = k == 0 ? nullptr : (k == 1 ? x0 : ring + (size_t)((k - 2) % 3) * nb);
   VT* xkp1 = ring + (size_t)(k % 3) * nb;
-  cheb_spmm_block<VT, B, V, kKpm, true, false, false>(
+  cheb_spmm_block<VT, B, V, kKpm, true, 0, false>(
       gb - e.blk0, e.ia, e.ja, nullptr, static_cast<const VT*>(e.a),
       reinterpret_cast<const int2*>(e.rbp), xk, xkm1, xkp1, nullptr, x0, nullptr, nullptr,
       k == 0 ? e.t0 : e.t, e.cc, 0, 0, e.part);
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kWG) void k_cheb_group(const ChebGroupEntry* __rest
   VT* xkp1 = ring + (size_t)((k - 1) % 3) * nb;
   const VT* xk = k == 1 ? x0 : ring + (size_t)((k - 2) % 3) * nb;
   const VT* xkm1 = k == 1 ? nullptr : (k == 2 ? x0 : ring + (size_t)((k - 3) % 3) * nb);
-  cheb_spmm_block<VT, B, P::V, kCheb, true, I16, false>(
+  cheb_spmm_block<VT, B, P::V, kCheb, true, I16 ? 1 : 0, false>(
       gb - e.blk0, e.ia, e.ja, e.jd, static_cast<const VT*>(e.a),
       reinterpret_cast<const int2*>(e.rbp), xk, xkm1, xkp1, e.y, nullptr,
       e.mus + (size_t)k * B, e.mus, k == 1 ? e.t1 : e.t2, e.cc, k == 1, y_flush_dev(k, e.deg),
@@ -751,6 +751,14 @@ __global__ void k_sum_splits(int nsplit, size_t cnt, const double* __restrict__ 
 }
 
 // out[k] = (a[k] - (ja[k] == row ? cc : 0)) * scale, one thread per row
+__global__ void k_shift_diag(int n, const double* __restrict__ d, double cc, double scale,
+                             float* __restrict__ out) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const double v = d[r] - cc;
+    out[r] = (float)(v * scale);
+  }
+}
+
 __global__ void k_shift_scale(int n, const int* __restrict__ ia, const int* __restrict__ ja,
                               const double* __restrict__ a, double cc, double scale,
                               float* __restrict__ out) {
@@ -1092,6 +1100,14 @@ hipError_t fill(size_t n, double v, double* x, hipStream_t st) {
 
 hipError_t fill_u32(size_t n, unsigned v, unsigned* x, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, kWG, 4096)), dim3(kWG), 0, st, n, v, x);
+  return hipGetLastError();
+}
+
+hipError_t shift_diag_f32(int n, const double* d, double cc, double scale, float* out,
+                          hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_shift_diag, dim3(grid_for(n, kWG, 4096)), dim3(kWG), 0, st, n, d, cc,
+                     scale, out);
   return hipGetLastError();
 }
 

@@ -100,7 +100,7 @@ int kpm_impl(int Mdeg, int damping, int nvec, const double* intv, int kind,
         EVSL_HIP(evsl_dev::sum_block_partials(red_part(), nblk_kpm, B,
                                               dots.as<double>() + (size_t)(k + 1) * B,
                                               red_scratch(), c.stream));
-        c.astats.filter_bytes += (double)c.A.nnz * (8 + c.A.idx_bytes()) + 4.0 * (n + 1) +
+        c.astats.filter_bytes += c.A.mat_bytes(8, B) + 4.0 * (n + 1) +
                                  (double)(k == 0 ? 3 : 4) * nb * sizeof(double);
       }
       c.astats.n_filter_steps += Mdeg;
@@ -357,7 +357,7 @@ int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, doub
     vbytes = 4.0 * vs;
   else
     vbytes = 3.0 * vs + 16.0 * flushes / nsteps + (mode == 3 ? vs : 0.0);
-  *bytes_per_step = (double)c.A.nnz * (vs + c.A.idx_bytes()) + 4.0 * (n + 1) + vbytes * nb;
+  *bytes_per_step = c.A.mat_bytes((int)vs, B) + 4.0 * (n + 1) + vbytes * nb;
   return 0;
 }
 
@@ -438,6 +438,24 @@ int evsl_amd_bench_dense(int kind, int n, int k, int m, int reps, double* ms_per
 int evsl_amd_spmm_index_bytes(void) {
   const Context& c = ctx();
   return c.op == 1 ? c.A.idx_bytes() : 0;
+}
+
+int evsl_amd_spmm_index_mode(int B, int prec, int mode) {
+  const Context& c = ctx();
+  if (c.op != 1 || B < 1) return -1;
+  const int vs = (prec == EVSL_PREC_MIXED && mode != 2) ? 4 : 8;
+  return (c.A.pc && B * vs <= evsl_dev::kPatMaxRowBytes) ? (c.A.pshort ? 2 : 3)
+                                                         : (c.A.jd ? 1 : 0);
+}
+
+int evsl_amd_set_pattern_codes(int on) {
+  ctx().pattern_codes = on != 0;
+  return 0;
+}
+
+int evsl_amd_pattern_codes(void) {
+  const Context& c = ctx();
+  return c.op == 1 && c.A.pc ? c.A.npat : 0;
 }
 
 int evsl_amd_spmm_layout(int B, int prec, int mode, int* decided) {

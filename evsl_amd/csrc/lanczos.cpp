@@ -79,7 +79,7 @@ struct EventClock {
 // matrix bytes of one launch (values of sa bytes) plus vec_bytes per row
 // and column of the B-column block
 double filter_step_bytes(const Context& c, int B, int sa, double vec_bytes) {
-  return (double)c.A.nnz * (sa + c.A.idx_bytes()) + 4.0 * (c.n + 1) + vec_bytes * B * c.n;
+  return c.A.mat_bytes((int)sa, B) + 4.0 * (c.n + 1) + vec_bytes * B * c.n;
 }
 
 }  // namespace

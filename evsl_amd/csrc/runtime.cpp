@@ -56,6 +56,16 @@ evsl_dev::Csr DevMatrix::view() const {
   c.jd = jd;
   c.a64 = a64;
   c.a32 = a32;
+  if (pc) {
+    c.pc = pc;
+    c.pdpos = pdpos;
+    c.poff = poff;
+    c.pshort = pshort;
+    c.pval64 = pval64;
+    c.dg64 = dg64;
+    c.pval32 = pval32;
+    c.dg32 = dg32;
+  }
   for (int w = 0; w < evsl_dev::kNumWidths; ++w) {
     c.rb[w] = rb[w];
     c.nblk[w] = nblk[w];
@@ -75,6 +85,9 @@ void DevMatrix::release() {
   if (jd) (void)hipFree(jd);
   if (a32) (void)hipFree(a32);
   if (a32s) (void)hipFree(a32s);
+  for (void* p : {(void*)pc, (void*)pdpos, (void*)poff, (void*)pval64, (void*)dg64, (void*)pval32,
+                  (void*)dg32, (void*)pval32s, (void*)dg32s})
+    if (p) (void)hipFree(p);
   evsl_dev::spmm_tuner_destroy(tune);
   *this = DevMatrix();
 }
@@ -134,6 +147,14 @@ int ensure_a32() {
   if (c.A.a32) return 0;
   EVSL_HIP(hipMalloc((void**)&c.A.a32, (size_t)std::max(c.A.nnz, 1) * sizeof(float)));
   EVSL_HIP(evsl_dev::f64_to_f32(c.A.nnz, c.A.a64, c.A.a32, c.stream));
+  if (c.A.pc) {  // the pattern tables in FP32: (float) of each value, as a32
+    std::vector<float> pv(evsl_dev::kPatCodes + 1, 0.0f);
+    for (int i = 0; i < c.A.npat; ++i) pv[i] = (float)c.A.pval_h[i];
+    EVSL_HIP(hipMalloc((void**)&c.A.pval32, pv.size() * sizeof(float)));
+    EVSL_HIP(hipMemcpy(c.A.pval32, pv.data(), pv.size() * sizeof(float), hipMemcpyHostToDevice));
+    EVSL_HIP(hipMalloc((void**)&c.A.dg32, (size_t)std::max(c.A.n, 1) * sizeof(float)));
+    EVSL_HIP(evsl_dev::f64_to_f32(c.A.n, c.A.dg64, c.A.dg32, c.stream));
+  }
   return 0;
 }
 
@@ -159,6 +180,62 @@ int apply_A(const double* x, double* y) {
 // row-block tables of every block width, 16-bit column offsets when they fit,
 // and whether every row stores its diagonal; *red_need = the reduction
 // scratch (doubles) its launches need
+namespace {
+
+// Pattern codes of a host CSR: every off-diagonal entry's (column offset,
+// value) pair gets a code in [0, kPatCodes) in order of first appearance
+// (values told apart by their bits), the diagonal entry kPatDiag with its
+// value in dg and its position within the row in dpos.  False when there
+// are more than kPatCodes distinct pairs, a row stores its diagonal twice
+// or past its 254th entry.
+bool encode_pattern(int n, const int* ia, const int* ja, const double* a,
+                    std::vector<unsigned char>& code, std::vector<int>& off,
+                    std::vector<double>& val, std::vector<double>& dg,
+                    std::vector<unsigned char>& dpos) {
+  constexpr int kSlots = 1024;  // open addressing, four times the table
+  struct Slot {
+    long long off;
+    std::uint64_t bits;
+    int code;
+  };
+  std::vector<Slot> h(kSlots, Slot{0, 0, -1});
+  code.resize((size_t)std::max(ia[n], 1));
+  dg.assign((size_t)n, 0.0);
+  dpos.assign((size_t)n, evsl_dev::kPatDiag);
+  off.clear();
+  val.clear();
+  for (int r = 0; r < n; ++r) {
+    bool diag = false;
+    for (int k = ia[r]; k < ia[r + 1]; ++k) {
+      const long long o = (long long)ja[k] - r;
+      if (o == 0) {
+        if (diag || k - ia[r] >= evsl_dev::kPatDiag) return false;
+        diag = true;
+        dg[r] = a[k];
+        dpos[r] = (unsigned char)(k - ia[r]);
+        code[k] = evsl_dev::kPatDiag;
+        continue;
+      }
+      std::uint64_t bits;
+      std::memcpy(&bits, &a[k], sizeof(bits));
+      const std::uint64_t mix =
+          (std::uint64_t)o * 0x9E3779B97F4A7C15ull ^ (bits * 0xC2B2AE3D27D4EB4Full);
+      int s = (int)((mix >> 32) & (kSlots - 1));
+      while (h[s].code >= 0 && !(h[s].off == o && h[s].bits == bits)) s = (s + 1) & (kSlots - 1);
+      if (h[s].code < 0) {
+        if ((int)off.size() == evsl_dev::kPatCodes) return false;
+        h[s] = Slot{o, bits, (int)off.size()};
+        off.push_back((int)o);
+        val.push_back(a[k]);
+      }
+      code[k] = (unsigned char)h[s].code;
+    }
+  }
+  return true;
+}
+
+}  // namespace
+
 int build_device_matrix(int n, int nnz, int* ia_d, int* ja_d, double* a_d, const int* ia_h,
                         const int* ja_h, bool owns, DevMatrix& M, size_t* red_need) {
   M.release();
@@ -213,6 +290,34 @@ int build_device_matrix(int n, int nnz, int* ia_d, int* ja_d, double* a_d, const
         for (int k = ia_h[r]; k < ia_h[r + 1]; ++k) jd[k] = (short)(ja_h[k] - r);
       EVSL_HIP(hipMalloc((void**)&M.jd, (size_t)nnz * sizeof(short)));
       EVSL_HIP(hipMemcpy(M.jd, jd.data(), (size_t)nnz * sizeof(short), hipMemcpyHostToDevice));
+    }
+  }
+  // pattern codes when the off-diagonal entries take few distinct
+  // (offset, value) pairs: 1 byte per nonzero for the filter kernels
+  if (ctx().pattern_codes && nnz > 0) {
+    std::vector<double> a_h((size_t)nnz);
+    EVSL_HIP(hipMemcpy(a_h.data(), a_d, (size_t)nnz * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<unsigned char> code, dpos;
+    std::vector<int> off;
+    std::vector<double> val, dg;
+    if (encode_pattern(n, ia_h, ja_h, a_h.data(), code, off, val, dg, dpos)) {
+      M.npat = (int)off.size();
+      M.pshort = true;
+      for (int o : off) M.pshort = M.pshort && o >= -32768 && o <= 32767;
+      M.pval_h = val;
+      off.resize(evsl_dev::kPatCodes + 1, 0);
+      val.resize(evsl_dev::kPatCodes + 1, 0.0);
+      EVSL_HIP(hipMalloc((void**)&M.pc, (size_t)nnz));
+      EVSL_HIP(hipMemcpy(M.pc, code.data(), (size_t)nnz, hipMemcpyHostToDevice));
+      EVSL_HIP(hipMalloc((void**)&M.poff, off.size() * sizeof(int)));
+      EVSL_HIP(hipMemcpy(M.poff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+      EVSL_HIP(hipMalloc((void**)&M.pval64, val.size() * sizeof(double)));
+      EVSL_HIP(hipMemcpy(M.pval64, val.data(), val.size() * sizeof(double),
+                         hipMemcpyHostToDevice));
+      EVSL_HIP(hipMalloc((void**)&M.pdpos, (size_t)n));
+      EVSL_HIP(hipMemcpy(M.pdpos, dpos.data(), (size_t)n, hipMemcpyHostToDevice));
+      EVSL_HIP(hipMalloc((void**)&M.dg64, (size_t)n * sizeof(double)));
+      EVSL_HIP(hipMemcpy(M.dg64, dg.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice));
     }
   }
   size_t need = (size_t)evsl_dev::kRedBlocks * 64;
@@ -270,6 +375,18 @@ int ensure_a32_shifted(double cc, double scale, bool* ok) {
   if (!c.A.a32s)
     EVSL_HIP(hipMalloc((void**)&c.A.a32s, (size_t)std::max(c.A.nnz, 1) * sizeof(float)));
   EVSL_HIP(evsl_dev::shift_scale_f32(c.A.view(), cc, scale, c.A.a32s, c.stream));
+  if (c.A.pc) {
+    // the pattern tables of the shifted copy, with the arithmetic of
+    // shift_scale_f32: (float)(v * scale) off the diagonal (every table
+    // entry), (float)((d - cc) * scale) on it
+    std::vector<float> pv(evsl_dev::kPatCodes + 1, 0.0f);
+    for (int i = 0; i < c.A.npat; ++i) pv[i] = (float)(c.A.pval_h[i] * scale);
+    if (!c.A.pval32s)
+      EVSL_HIP(hipMalloc((void**)&c.A.pval32s, pv.size() * sizeof(float)));
+    EVSL_HIP(hipMemcpy(c.A.pval32s, pv.data(), pv.size() * sizeof(float), hipMemcpyHostToDevice));
+    if (!c.A.dg32s) EVSL_HIP(hipMalloc((void**)&c.A.dg32s, (size_t)std::max(c.A.n, 1) * sizeof(float)));
+    EVSL_HIP(evsl_dev::shift_diag_f32(c.A.n, c.A.dg64, cc, scale, c.A.dg32s, c.stream));
+  }
   c.A.shift_cc = cc;
   c.A.shift_scale = scale;
   *ok = true;
@@ -296,6 +413,8 @@ int make_filter_op(double cc, double dd, bool f32, FilterOp* op) {
   if (int e = ensure_a32_shifted(cc, scale, &ok)) return e;
   if (ok) {
     op->A.a32 = c.A.a32s;
+    op->A.pval32 = c.A.pval32s;
+    op->A.dg32 = c.A.dg32s;
     op->t1 /= scale;
     op->t2 /= scale;
     op->cc = 0.0;

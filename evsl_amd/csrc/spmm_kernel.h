@@ -52,18 +52,37 @@ template <typename T>
 struct ExtVec<T, 1> {
   typedef T type;
 };
+// Column indices of the filter kernels, by index mode IX:
+//   0: ja (int32 columns), 1: jd (16-bit offsets from the row),
+//   2, 3: pattern codes (Csr::pc), their table offsets staged as 16-bit (2)
+//   or 32-bit (3) offsets from the row.
+// Pattern tables of one precision (null pc: not in use).
+struct PatArgs {
+  const unsigned char* pc;
+  const int* off;
+  const void* val;
+  const void* dg;
+  const unsigned char* dpos;  // position of the diagonal in its row (kPatDiag: none)
+};
+
 // The work of one row block `blk`: shared by the one-matrix kernel
-// (k_cheb_spmm) and the grouped KPM kernel (k_kpm_group), so that a block's
-// arithmetic is the same in both.
-template <typename VT, int B, int V, int MODE, bool LDS, bool I16, bool NT>
+// (k_cheb_spmm) and the grouped kernels (k_kpm_group, k_cheb_group), so that
+// a block's arithmetic is the same in all of them.
+template <typename VT, int B, int V, int MODE, bool LDS, int IX, bool NT>
 __device__ __forceinline__ void cheb_spmm_block(
     const int blk, const int* __restrict__ ia, const int* __restrict__ ja,
     const short* __restrict__ jd, const VT* __restrict__ av, const int2* __restrict__ rbp,
     const VT* __restrict__ xk, const VT* __restrict__ xkm1, VT* __restrict__ xkp1,
     double* __restrict__ y, const VT* __restrict__ v0, const double* __restrict__ scol,
     const double* __restrict__ mu0col, double t_, double cc_, int first, int yflush,
-    double* __restrict__ part, int* s_ia = nullptr) {
+    double* __restrict__ part, int* s_ia = nullptr, PatArgs pat = PatArgs{}) {
   static_assert(B % V == 0, "V must divide B");
+  static_assert(IX >= 0 && IX <= 3, "index mode");
+  static_assert(IX < 2 || LDS, "pattern codes are decoded at staging");
+  constexpr bool kRel = IX != 0;  // staged columns are offsets from the row
+  constexpr bool kPat = IX >= 2;  // one-byte pattern codes
+  const VT* __restrict__ pval = static_cast<const VT*>(pat.val);
+  const VT* __restrict__ pdg = static_cast<const VT*>(pat.dg);
   constexpr int L = B / V;
   constexpr int NG = kWG / L;  // lanes >= NG*L idle when L does not divide 256
   constexpr int U = (V * sizeof(VT) >= 32) ? 4 : 8;  // gathers in flight per lane
@@ -71,12 +90,13 @@ __device__ __forceinline__ void cheb_spmm_block(
   using VecY = VecT<double, V>;
   // staging area for ja/a, reused for the group reductions once the row
   // walk is over
-  // column indices: int32, or (I16) 16-bit offsets from the row, ja = row + jd
-  using IT = typename std::conditional<I16, short, int>::type;
-  const IT* __restrict__ jx;
-  if constexpr (I16)
+  // column indices: int32, or 16-bit offsets from the row, ja = row + jd
+  // (the pattern modes stage their table offsets the same way)
+  using IT = typename std::conditional<IX == 1 || IX == 2, short, int>::type;
+  const IT* __restrict__ jx = nullptr;
+  if constexpr (IX == 1)
     jx = jd;
-  else
+  else if constexpr (IX == 0)
     jx = ja;
   constexpr size_t kStage = LDS ? (size_t)kNnzCap * (sizeof(IT) + sizeof(VT)) : 0;
   constexpr size_t kRed = (size_t)kWG * V * sizeof(double);
@@ -271,12 +291,51 @@ __device__ __forceinline__ void cheb_spmm_block(
       IT jv[kStageIt];
       VT avv[kStageIt];
       int iv[2] = {0, 0};
+      // pattern modes: thread t also places row r0 + t's diagonal value
+      int dsl = -1;
+      VT dval = 0;
+      if constexpr (kPat) {
+        // one-byte codes, then their (offset, value) from the tables, which
+        // the block first copies to LDS (their loads go out beside the
+        // code loads); a diagonal code stages offset 0, and its value is
+        // written by the thread of its row (position within the row from
+        // dpos)
+        __shared__ IT s_toff[kPatCodes + 1];
+        __shared__ VT s_tval[kPatCodes + 1];
+        unsigned char cv[kStageIt];
 #pragma unroll
-      for (int u = 0; u < kStageIt; ++u) {
-        const int p = tid + u * kWG;
-        if (p < nz) {
-          jv[u] = jx[p0 + p];
-          avv[u] = av[p0 + p];
+        for (int u = 0; u < kStageIt; ++u) {
+          const int p = tid + u * kWG;
+          cv[u] = p < nz ? pat.pc[p0 + p] : kPatDiag;
+        }
+        static_assert(kPatCodes + 1 <= kWG, "one table entry per thread");
+        if (tid <= kPatCodes) {
+          s_toff[tid] = (IT)pat.off[tid];
+          s_tval[tid] = pval[tid];
+        }
+        if (tid < r1 - r0) {
+          const int r = r0 + tid;
+          const unsigned char dp = pat.dpos[r];
+          if (dp != kPatDiag) {
+            dsl = ia[r] - p0 + dp;
+            dval = pdg[r];
+          }
+        }
+        __syncthreads();  // the tables
+#pragma unroll
+        for (int u = 0; u < kStageIt; ++u) {
+          const bool d = cv[u] == kPatDiag;
+          jv[u] = d ? (IT)0 : s_toff[cv[u]];
+          avv[u] = d ? (VT)0 : s_tval[cv[u]];
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < kStageIt; ++u) {
+          const int p = tid + u * kWG;
+          if (p < nz) {
+            jv[u] = jx[p0 + p];
+            avv[u] = av[p0 + p];
+          }
         }
       }
       if (ia_lds) {
@@ -289,9 +348,10 @@ __device__ __forceinline__ void cheb_spmm_block(
         const int p = tid + u * kWG;
         if (p < nz) {
           s_ja[p] = jv[u];
-          s_a[p] = avv[u];
+          if (!kPat || jv[u] != (IT)0) s_a[p] = avv[u];
         }
       }
+      if (kPat && dsl >= 0) s_a[dsl] = dval;
       if (ia_lds) {
 #pragma unroll
         for (int u = 0; u < 2; ++u)
@@ -330,7 +390,7 @@ __device__ __forceinline__ void cheb_spmm_block(
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const bool ok = q + u < q1;
-          jj[u] = ok ? (I16 ? r : 0) + (int)jsrc[q + u] : 0;
+          jj[u] = ok ? (kRel ? r : 0) + (int)jsrc[q + u] : 0;
           aa[u] = ok ? asrc[q + u] : (VT)0;
         }
 #pragma unroll
@@ -362,8 +422,18 @@ __device__ __forceinline__ void cheb_spmm_block(
 #pragma unroll
       for (int j = 0; j < V; ++j) acc[j] = 0;
       for (int p = p0 + st; p < p1; p += kLongStripes) {
-        const VT a = av[p];
-        const Vec x = ld(xk, (size_t)((I16 ? r0 : 0) + (int)jx[p]) * B + c0);
+        VT a;
+        int col;
+        if constexpr (kPat) {
+          const unsigned char cd = pat.pc[p];
+          const bool d = cd == kPatDiag;
+          a = d ? pdg[r0] : pval[cd];
+          col = r0 + (d ? 0 : pat.off[cd]);
+        } else {
+          a = av[p];
+          col = (kRel ? r0 : 0) + (int)jx[p];
+        }
+        const Vec x = ld(xk, (size_t)col * B + c0);
 #pragma unroll
         for (int j = 0; j < V; ++j) acc[j] += a * x.v[j];
       }
@@ -405,7 +475,7 @@ __device__ __forceinline__ void cheb_spmm_block(
   }
 }
 
-template <typename VT, int B, int V, int MODE, bool LDS, bool I16, bool NT = false>
+template <typename VT, int B, int V, int MODE, bool LDS, int IX, bool NT = false>
 __global__ __launch_bounds__(kWG) void k_cheb_spmm(
     const int* __restrict__ ia, const int* __restrict__ ja, const short* __restrict__ jd,
     const VT* __restrict__ av,
@@ -413,11 +483,26 @@ __global__ __launch_bounds__(kWG) void k_cheb_spmm(
     const VT* __restrict__ xkm1, VT* __restrict__ xkp1, double* __restrict__ y,
     const VT* __restrict__ v0, const double* __restrict__ scol,
     const double* __restrict__ mu0col, double t_, double cc_, int first, int yflush,
-    double* __restrict__ part, int stage_ia) {
+    double* __restrict__ part, int stage_ia, PatArgs pat) {
   extern __shared__ int s_dyn[];  // kRowsCap + 1 ints when stage_ia
-  cheb_spmm_block<VT, B, V, MODE, LDS, I16, NT>(xcd_remap(blockIdx.x, nblk), ia, ja, jd, av, rbp,
-                                                xk, xkm1, xkp1, y, v0, scol, mu0col, t_, cc_,
-                                                first, yflush, part, stage_ia ? s_dyn : nullptr);
+  cheb_spmm_block<VT, B, V, MODE, LDS, IX, NT>(xcd_remap(blockIdx.x, nblk), ia, ja, jd, av, rbp,
+                                               xk, xkm1, xkp1, y, v0, scol, mu0col, t_, cc_,
+                                               first, yflush, part, stage_ia ? s_dyn : nullptr,
+                                               pat);
+}
+
+// the index mode of a matrix's filter launches at width B
+inline int index_mode(const Csr& A, int B, int vsize) {
+  return pattern_launch(A, B, vsize) ? (A.pshort ? 2 : 3) : (A.jd ? 1 : 0);
+}
+
+template <typename VT>
+inline PatArgs pat_args(const Csr& A, int B) {
+  if (!pattern_launch(A, B, (int)sizeof(VT))) return PatArgs{};
+  if constexpr (sizeof(VT) == 4)
+    return PatArgs{A.pc, A.poff, A.pval32, A.dg32, A.pdpos};
+  else
+    return PatArgs{A.pc, A.poff, A.pval64, A.dg64, A.pdpos};
 }
 
 // Non-temporal streams (the NT instances) exist for the ChebAv and residual
@@ -438,8 +523,9 @@ inline bool streaming_launch(const Csr& A, int B, int vsize, int mode) {
   if (!((mode == kCheb || mode == kRes) && B >= 8)) return false;
   // xk, xkm1, xkp1 (and R) of the vector type, y in FP64
   const double streams = mode == kRes ? 4.0 : 3.0;
-  const double ws = streams * (double)A.n * B * vsize + 8.0 * A.n * B +
-                    (double)A.nnz * (vsize + (A.jd ? 2 : 4)) + 4.0 * (A.n + 1);
+  const double mat = pattern_launch(A, B, vsize) ? (double)A.nnz + (double)A.n * (vsize + 1)
+                          : (double)A.nnz * (vsize + (A.jd ? 2 : 4));
+  const double ws = streams * (double)A.n * B * vsize + 8.0 * A.n * B + mat + 4.0 * (A.n + 1);
   return ws > 320e6;
 }
 
@@ -465,23 +551,36 @@ hipError_t launch_variant(const Csr& A, const void* xk, const void* xkm1, void* 
   // lane groups walk R rows: the row pointers go to dynamic LDS
   const int stage_ia = (LDS && R > 1) ? 1 : 0;
   const size_t dyn = stage_ia ? (size_t)(kRowsCap + 1) * sizeof(int) : 0;
-#define EVSL_SPMM_LAUNCH(I16_, NT_)                                                             \
-  hipLaunchKernelGGL((k_cheb_spmm<VT, B, V, MODE, LDS, I16_, NT_>), dim3(nblk), dim3(kWG), dyn, \
-                     st, A.ia, A.ja, A.jd, av, rb, nblk, (const VT*)xk, (const VT*)xkm1,      \
-                     (VT*)xkp1, (double*)y, (const VT*)v0, s, mu0, t, cc, f, yflush, part, stage_ia)
+  const PatArgs pat = pat_args<VT>(A, B);
+  if (pat.pc && (!pat.val || !pat.dg)) return hipErrorInvalidValue;
+  const int ix = index_mode(A, B, (int)sizeof(VT));
+#define EVSL_SPMM_LAUNCH(IX_, NT_)                                                             \
+  hipLaunchKernelGGL((k_cheb_spmm<VT, B, V, MODE, LDS, IX_, NT_>), dim3(nblk), dim3(kWG), dyn, \
+                     st, A.ia, A.ja, A.jd, av, rb, nblk, (const VT*)xk, (const VT*)xkm1,     \
+                     (VT*)xkp1, (double*)y, (const VT*)v0, s, mu0, t, cc, f, yflush, part,    \
+                     stage_ia, pat)
+#define EVSL_SPMM_BY_IX(NT_)    \
+  switch (ix) {                 \
+    case 3:                     \
+      EVSL_SPMM_LAUNCH(3, NT_); \
+      break;                    \
+    case 2:                     \
+      EVSL_SPMM_LAUNCH(2, NT_); \
+      break;                    \
+    case 1:                     \
+      EVSL_SPMM_LAUNCH(1, NT_); \
+      break;                    \
+    default:                    \
+      EVSL_SPMM_LAUNCH(0, NT_); \
+  }
   if constexpr (kStreamingCapable<MODE, B>) {
     if (streaming_launch(A, B, (int)sizeof(VT), MODE)) {
-      if (A.jd)
-        EVSL_SPMM_LAUNCH(true, true);
-      else
-        EVSL_SPMM_LAUNCH(false, true);
+      EVSL_SPMM_BY_IX(true)
       return hipGetLastError();
     }
   }
-  if (A.jd)
-    EVSL_SPMM_LAUNCH(true, false);
-  else
-    EVSL_SPMM_LAUNCH(false, false);
+  EVSL_SPMM_BY_IX(false)
+#undef EVSL_SPMM_BY_IX
 #undef EVSL_SPMM_LAUNCH
   return hipGetLastError();
 }

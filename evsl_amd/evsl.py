@@ -42,7 +42,8 @@ EXPORTS = [
     "evsl_amd_get_stats", "evsl_amd_reset_stats",
     "evsl_amd_last_error", "evsl_amd_stats_size", "kpmdos_ex", "evsl_amd_bench_filter",
     "ChebLanNr_batch", "evsl_amd_read_coo_mm", "evsl_amd_bench_copy", "evsl_amd_bench_dense",
-    "evsl_amd_spmm_vec", "evsl_amd_spmm_index_bytes", "evsl_amd_cheb_av_cols",
+    "evsl_amd_spmm_vec", "evsl_amd_spmm_index_bytes", "evsl_amd_spmm_index_mode",
+    "evsl_amd_set_pattern_codes", "evsl_amd_pattern_codes", "evsl_amd_cheb_av_cols",
     "evsl_amd_cheb_av_cols_noalloc", "evsl_amd_set_grouped_filter",
     "evsl_amd_set_mixed_capped_fp64",
     "evsl_amd_spmm_streaming", "kpmdos_batch",
@@ -139,6 +140,9 @@ class EVSL:
             "evsl_amd_set_spmm_layout": (I, [I, I, I, I]),
             "evsl_amd_set_spmm_tuning": (I, [I]),
             "evsl_amd_spmm_index_bytes": (I, []),
+            "evsl_amd_spmm_index_mode": (I, [I, I, I]),
+            "evsl_amd_set_pattern_codes": (I, [I]),
+            "evsl_amd_pattern_codes": (I, []),
             "evsl_amd_spmm_streaming": (I, [I, I, I]),
             "kpmdos_batch": (I, [I, C.POINTER(C.POINTER(CsrMat)), dp, I, I, I, I,
                                  C.POINTER(C.c_ulonglong), I, dp, dp]),
@@ -461,10 +465,16 @@ class EVSL:
         """Template name of the fused filter kernel for (B, prec, mode) on
         the current matrix, as rocprofv3 reports it."""
         vt = "float" if prec == PREC_MIXED and mode != 2 else "double"
-        i16 = "true" if self.lib.evsl_amd_spmm_index_bytes() == 2 else "false"
+        ix = self.lib.evsl_amd_spmm_index_mode(B, prec, mode)
         nt = "true" if self.lib.evsl_amd_spmm_streaming(B, prec, mode) else "false"
-        return (f"k_cheb_spmm<{vt}, {B}, {self.spmm_vec(B, prec, mode)}, {mode}, true, {i16}, "
+        return (f"k_cheb_spmm<{vt}, {B}, {self.spmm_vec(B, prec, mode)}, {mode}, true, {ix}, "
                 f"{nt}>")
+
+    def set_pattern_codes(self, on):
+        return self.lib.evsl_amd_set_pattern_codes(1 if on else 0)
+
+    def pattern_codes(self):
+        return self.lib.evsl_amd_pattern_codes()
 
 
 class Server:

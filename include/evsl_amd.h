@@ -240,18 +240,41 @@ int evsl_amd_set_spmm_tuning(int on);
  * for (B, prec, mode) on the current matrix (the tuned layout's V, after the
  * tuner has settled; bench_filter lets it settle before timing): its
  * template instance is
- * k_cheb_spmm<float|double, B, V, mode, true, I16, NT> (I16 from
- * evsl_amd_spmm_index_bytes, NT from evsl_amd_spmm_streaming); -1 for an
+ * k_cheb_spmm<float|double, B, V, mode, true, IX, NT> (IX from
+ * evsl_amd_spmm_index_mode, NT from evsl_amd_spmm_streaming); -1 for an
  * unsupported B. */
 int evsl_amd_spmm_vec(int B, int prec, int mode);
 /* 1 if the fused filter kernel for (B, prec, mode) on the current matrix
- * runs its non-temporal instance (k_cheb_spmm<..., I16, true>: one launch's
+ * runs its non-temporal instance (k_cheb_spmm<..., IX, true>: one launch's
  * working set exceeds 320 MB), 0 otherwise or without a matrix. */
 int evsl_amd_spmm_streaming(int B, int prec, int mode);
-/* Bytes per nonzero of the column index the filter kernels read for the
- * current matrix: 2 when every |ja - row| fits in 16 bits (the kernels then
- * run as k_cheb_spmm<..., true, true>), 4 otherwise; 0 without a matrix. */
+/* Bytes per nonzero of the column index of the current matrix: 1 with
+ * pattern codes (below), 2 when every |ja - row| fits in 16 bits, 4
+ * otherwise; 0 without a matrix. */
 int evsl_amd_spmm_index_bytes(void);
+/* Index mode of the current matrix's fused filter launches for (B, prec,
+ * mode) (the IX of the kernel instance): 0 int32 columns, 1 16-bit column
+ * offsets, 2 and 3 pattern codes whose table offsets fit 16 bits (2) or
+ * not (3) -- pattern codes are read by launches whose vector rows hold at
+ * most 8 bytes (one or two FP32 columns, one FP64 column); -1 without a
+ * matrix. */
+int evsl_amd_spmm_index_mode(int B, int prec, int mode);
+/* Pattern codes (no reference counterpart): a matrix whose off-diagonal
+ * entries take at most 255 distinct (column offset, value) pairs -- stencil
+ * and lattice operators -- is stored for the filter kernels as one byte
+ * per nonzero naming its pair (the diagonal by a code of its own, its value
+ * from a per-row array) beside the CSR arrays; the kernels then stream 1
+ * byte per nonzero instead of an index and a value, with the same products
+ * in the same order, so results do not change.
+ *   evsl_amd_set_pattern_codes: 1 (the default) uses them for matrices set
+ *     from now on, 0 does not.
+ *   evsl_amd_pattern_codes: table entries in use for the current matrix,
+ *     0 when it is not pattern-coded.
+ * Launches whose vector rows hold more than 8 bytes read the index and
+ * value arrays instead (evsl_amd_spmm_index_mode): there the table lookup
+ * ahead of the gathers costs more than the bytes it saves. */
+int evsl_amd_set_pattern_codes(int on);
+int evsl_amd_pattern_codes(void);
 
 /* Measurement hook: mean time of a device copy of `bytes` (16-byte
  * accesses) over reps launches -- the achievable-bandwidth reference

@@ -26,11 +26,13 @@ WIDTHS = [1, 2, 3, 4, 5, 6, 7, 8, 11, 16, 23, 32, 40]
 
 
 def make_case(name):
-    if name == "lap3d_i16":  # every |ja - row| <= 256: 16-bit column offsets
+    if name in ("lap3d_i16", "lap3d_pattern"):
+        # every |ja - row| <= 256: 16-bit column offsets, or with pattern
+        # codes one byte per nonzero (7 (offset, value) pairs)
         A = M.perturb_diag(M.laplacian(16, 16, 16), 1e-3, seed=9)
         lo, hi = 0.0, 12.001
         ivs = [(0.4, 0.7), (1.5, 1.9), (5.0, 6.0)]
-        ibytes = 2
+        ibytes = 2 if name == "lap3d_i16" else 1
     else:  # columns anywhere in [0, n): 32-bit column indices
         A = M.random_symmetric(40000, 8, 32, seed=6)
         lo, hi = M.gershgorin(A)
@@ -53,10 +55,14 @@ def mu_of(pol):
     return np.array([pol.mu[k] for k in range(pol.deg + 1)])
 
 
-@pytest.mark.parametrize("case", ["lap3d_i16", "randsym_i32"])
+@pytest.mark.parametrize("case", ["lap3d_i16", "lap3d_pattern", "randsym_i32"])
 def test_filter_every_width(ev, oracle, case):
     A, lo, hi, ivs, ibytes = make_case(case)
-    assert ev.set_matrix(A) == 0, ev.last_error()
+    ev.set_pattern_codes(case != "lap3d_i16")
+    try:
+        assert ev.set_matrix(A) == 0, ev.last_error()
+    finally:
+        ev.set_pattern_codes(1)
     assert ev.lib.evsl_amd_spmm_index_bytes() == ibytes
     oracle.set_csr(A)
     pg, po = make_pols(ev, ivs, lo, hi), make_pols(oracle, ivs, lo, hi)

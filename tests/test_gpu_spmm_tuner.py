@@ -69,9 +69,10 @@ def test_tuner_candidates_and_argument_errors(ev):
     reset_keys(ev)
 
 
-@pytest.mark.parametrize("case", ["lap3d_i16", "anderson_periodic", "randsym_i32"])
+@pytest.mark.parametrize("case", ["lap3d_i16", "lap3d_pattern", "anderson_periodic",
+                                  "randsym_i32"])
 def test_every_layout_same_bits_chebav(ev, case):
-    if case == "lap3d_i16":
+    if case in ("lap3d_i16", "lap3d_pattern"):
         A = M.perturb_diag(M.laplacian(24, 24, 24), 1e-3, seed=2)
         lo, hi = 0.0, 12.001
         ivs = [(0.4, 0.7), (1.5, 1.9), (5.0, 6.0)]
@@ -83,7 +84,13 @@ def test_every_layout_same_bits_chebav(ev, case):
         A = M.random_symmetric(20000, 8, 32, seed=12)
         lo, hi = M.gershgorin(A)
         ivs = [(-0.3, 0.3), (1.0, 2.0), (4.0, 6.0)]
-    assert ev.set_matrix(A) == 0, ev.last_error()
+    ev.set_pattern_codes(case != "lap3d_i16")
+    try:
+        assert ev.set_matrix(A) == 0, ev.last_error()
+    finally:
+        ev.set_pattern_codes(1)
+    # the stencil and lattice matrices run pattern codes unless switched off
+    assert (ev.pattern_codes() > 0) == (case in ("lap3d_pattern", "anderson_periodic"))
     pg = make_pols(ev, ivs, lo, hi)
     X = np.random.default_rng(3).uniform(-1.0, 1.0, (A.n, 32))
     # one polynomial for all columns: every launch of a call runs at width B
