@@ -512,7 +512,7 @@ int residual_filter(int n, int m, double* U, double* W, const polparams* pol, Bl
 int refine_extract(int n, int kdim, const double* dT, const double* eT, const double* V,
                    const polparams* pol, const double* intv, double tol, bool mixed,
                    int* nevOut, double** lamo, double** Wo, double** reso, FILE* fstats,
-                   bool* stalled) {
+                   bool* stalled, double guard) {
   Context& c = ctx();
   if (stalled) *stalled = false;
   const double bar = pol->bar, aa = intv[0], bb = intv[1];
@@ -523,7 +523,7 @@ int refine_extract(int n, int kdim, const double* dT, const double* eT, const do
   if (int err = tridiag_ql(kdim, eval.data(), e.data(), &rec)) return err;
   // guard threshold (default 0.6 bar): keep Ritz values of p(A) so that
   // eigenvectors just outside [a, b] are in the Rayleigh-Ritz space
-  const double thr = mixed ? c.refine_guard * bar : bar;
+  const double thr = mixed ? (guard > 0.0 ? guard : c.refine_guard) * bar : bar;
   std::vector<int> sel;
   for (int i = 0; i < kdim; ++i)
     if (!(eval[i] < thr)) sel.push_back(i);
@@ -1162,7 +1162,21 @@ int LanczosGroup::step_done() {
       bool stalled = false;
       int err = refine_extract(n, L.kdim, L.dT.data(), L.eT.data(), L.V.as<double>(), L.pol,
                                L.intv, sl.tol, sl.f32, &o->nev, &o->lam, &o->W, &o->res, fstats,
-                               &stalled);
+                               &stalled, -1.0);
+      if (!err && stalled) {
+        // once more with half the guard threshold: the eigenvectors whose
+        // components a pass damps least lie just below the guard set, so a
+        // wider Rayleigh-Ritz space takes them out
+        if (fstats) std::fprintf(fstats, "slice %d: refinement stalled, again at half the guard\n", s);
+        std::free(o->lam);
+        std::free(o->res);
+        if (o->W) _evsl_Free_device(o->W);
+        *o = SliceResult();
+        stalled = false;
+        err = refine_extract(n, L.kdim, L.dT.data(), L.eT.data(), L.V.as<double>(), L.pol,
+                             L.intv, sl.tol, sl.f32, &o->nev, &o->lam, &o->W, &o->res, fstats,
+                             &stalled, 0.5 * c.refine_guard);
+      }
       c.astats.t_ritz += evsl_timer() - tr;
       if (!err && stalled) {
         // the FP32-filtered basis cannot reach tol: the slice again in FP64

@@ -74,6 +74,7 @@ int evsl_amd_set_refine_widen(double widen);
  * and its slice tends to end at maxit, where the FP32-filtered basis leaves
  * pairs just above tol (both counted in evslAmdStats.n_fp64_slices).  A
  * mixed slice whose refinement stalls above tol after its extra passes is
+ * refined once more with half the guard threshold, and if that stalls too,
  * redone from its start vector in FP64 (evslAmdStats.n_fp64_reruns). */
 int evsl_amd_set_mixed_degree_range(int min_deg, int max_deg);
 /* MIXED mode: whether a capped filter (deg = max_deg - 1) runs in FP64 (1,
