@@ -521,7 +521,7 @@ int refine_extract(int n, int kdim, const double* dT, const double* eT, const do
   if (kdim > 1) std::memcpy(e.data(), eT, (kdim - 1) * sizeof(double));
   RotationLog rec;
   if (int err = tridiag_ql(kdim, eval.data(), e.data(), &rec)) return err;
-  // guard threshold (default 0.6 bar): keep Ritz values of p(A) so that
+  // guard threshold (default 0.8 bar): keep Ritz values of p(A) so that
   // eigenvectors just outside [a, b] are in the Rayleigh-Ritz space
   const double thr = mixed ? (guard > 0.0 ? guard : c.refine_guard) * bar : bar;
   std::vector<int> sel;

@@ -112,11 +112,11 @@ struct Context {
   hipStream_t stream = nullptr;  // the legacy null stream (serialises with user code)
   int filter_prec = EVSL_PREC_FP64;
   int refine_passes = 1;
-  double refine_guard = 0.6;
+  double refine_guard = 0.8;
   // the first refinement pass filters with the polynomial of the slice's
   // interval widened by refine_widen slice widths on each side (lower
   // degree); 0 uses the slice's own filter
-  double refine_widen = 2.0;
+  double refine_widen = 4.0;
   int mixed_min_deg = 32;    // mixed form: filters of degree outside
   int mixed_max_deg = 20000; // [mixed_min_deg, mixed_max_deg] run in FP64
   bool mixed_capped_fp64 = true;  // mixed form: capped filters run in FP64

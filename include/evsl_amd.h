@@ -57,14 +57,14 @@ int evsl_amd_get_filter_precision(void);
 int evsl_amd_set_refine_passes(int passes);
 /* guard threshold of the refinement: Ritz values of p(A) down to
  * guard * bar enter the FP64 Rayleigh-Ritz space (0 < guard <= 1,
- * default 0.6) */
+ * default 0.8; a stalled refinement is retried once at half of it) */
 int evsl_amd_set_refine_guard(double guard);
 /* the first refinement pass filters with find_pol's polynomial for the
  * slice's interval widened by `widen` slice widths on each side (clipped to
  * [lmin, lmax]): a lower degree, enough to damp the FP32 filter's rounding
  * noise far from the interval, which is what the residuals weigh most;
  * further passes, while a Ritz value in [a, b] is near convergence, use the
- * slice's own filter.  0 makes every pass use the slice's filter (default 2) */
+ * slice's own filter.  0 makes every pass use the slice's filter (default 4) */
 int evsl_amd_set_refine_widen(double widen);
 /* MIXED mode: only slices whose filter degree lies in [min_deg, max_deg]
  * filter in FP32 (default [32, 20000]); the others run in FP64 -- a
