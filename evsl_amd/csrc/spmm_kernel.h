@@ -309,6 +309,7 @@ __device__ __forceinline__ void cheb_spmm_block(
           cv[u] = p < nz ? pat.pc[p0 + p] : kPatDiag;
         }
         static_assert(kPatCodes + 1 <= kWG, "one table entry per thread");
+        static_assert(kRowsCap <= kWG, "one row of the block per thread");
         if (tid <= kPatCodes) {
           s_toff[tid] = (IT)pat.off[tid];
           s_tval[tid] = pval[tid];
