@@ -682,6 +682,8 @@ def c3_bench(args):
                       "max_residual": float(max((o[2].max() for o in outs if len(o[2])),
                                                 default=0.0)),
                       "lanczos_steps": int(st["n_lanczos"]),
+                      "refinements_retried": int(st["n_refine_retries"]),
+                      "slices_rerun_in_fp64": int(st["n_fp64_reruns"]),
                       "breakdown_s": {"filter": st["t_filter"], "reorth": st["t_reorth"],
                                       "ritz_refine": st["t_ritz"]}}))
     for p in pols:
@@ -902,7 +904,8 @@ def main():
                    "refine_extra_passes": int(st["n_refine_extra"]),
                    "refine_GB": st["refine_bytes"] / 1e9,
                    "slices_fp64_under_mixed": int(st["n_fp64_slices"]),
-                   "slices_rerun_in_fp64": int(st["n_fp64_reruns"])}
+                   "slices_rerun_in_fp64": int(st["n_fp64_reruns"]),
+                   "refinements_retried": int(st["n_refine_retries"])}
         if args.form != "plain" else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

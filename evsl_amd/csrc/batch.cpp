@@ -1168,6 +1168,7 @@ int LanczosGroup::step_done() {
         // components a pass damps least lie just below the guard set, so a
         // wider Rayleigh-Ritz space takes them out
         if (fstats) std::fprintf(fstats, "slice %d: refinement stalled, again at half the guard\n", s);
+        ++c.astats.n_refine_retries;
         std::free(o->lam);
         std::free(o->res);
         if (o->W) _evsl_Free_device(o->W);

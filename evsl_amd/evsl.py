@@ -63,7 +63,8 @@ class AmdStats(C.Structure):
                 ("reorth_bytes", C.c_double), ("n_refine_cols", C.c_size_t),
                 ("n_refine_extra", C.c_size_t), ("refine_bytes", C.c_double),
                 ("n_refine_degcols", C.c_size_t), ("n_fp64_slices", C.c_size_t),
-                ("n_fp64_reruns", C.c_size_t), ("n_refine_fp64_cols", C.c_size_t)]
+                ("n_fp64_reruns", C.c_size_t), ("n_refine_fp64_cols", C.c_size_t),
+                ("n_refine_retries", C.c_size_t)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -42,6 +42,8 @@ typedef struct _evslAmdStats {
                          /* refinement                                     */
   size_t n_refine_fp64_cols; /* refinement columns filtered in FP64 (their */
                              /* residual too large for the FP32 recurrence) */
+  size_t n_refine_retries; /* MIXED: stalled refinements run again at half  */
+                           /* the guard threshold                           */
 } evslAmdStats;
 
 #ifdef __cplusplus

@@ -264,3 +264,56 @@ def test_mixed_fp32_filter_at_high_degree_anderson(ev):
         assert np.max(res) < tol
     for p in pols:
         ev.free_pol(p)
+
+
+def test_mixed_stalled_refinement_retried_at_half_guard(ev):
+    """A refinement that stops converging is run once more at half the guard
+    threshold before the slice would be redone in FP64.  Four band-centre
+    slices of a 3-D Anderson model (24^3 periodic, W = 16.5, filter degrees
+    ~1800) with the guard raised to 0.99: one slice's refinement stalls,
+    is retried at 0.495 and converges there -- no slice is redone in FP64,
+    the per-slice counts equal the FP64 run's, eigenvalues agree with it to
+    1e-10 and residuals stay below tol."""
+    A = M.anderson(24, 16.5, seed=5)
+    lo, hi = M.gershgorin(A)
+    assert ev.set_matrix(A) == 0
+    intv = np.array([-0.05, 0.05, lo, hi])
+    ev.set_precision(PREC_FP64)
+    err, mu, ecnt = ev.kpmdos(300, 1, 40, intv, vec_kind=VEC_RADEMACHER, seed=7)
+    assert err == 0
+    err, sli = ev.spslicer(mu, intv, 4, int(10 * ecnt) + 50)
+    assert err == 0
+    mlan = min(max(5 * (int(1 + ecnt / 4) + 2), 300), A.n)
+    ivs, pols = [], []
+    for k in range(4):
+        iv = np.array([sli[k], sli[k + 1], lo, hi])
+        pol = ev.default_pol()
+        pol.damping, pol.thresh_int, pol.thresh_ext, pol.max_deg = 2, 0.8, 0.2, 10000
+        assert ev.find_pol(iv, pol) == 0
+        ivs.append(iv)
+        pols.append(pol)
+    v0 = np.random.default_rng(3).uniform(-1, 1, A.n)
+    tol = 1e-8
+    try:
+        e64, r64 = ev.ChebLanNr_batch(ivs, mlan, tol, v0, pols)
+        assert e64 == 0, ev.last_error()
+        ev.set_precision(PREC_MIXED)
+        ev.set_refine_guard(0.99)
+        ev.reset_stats()
+        em, rm = ev.ChebLanNr_batch(ivs, mlan, tol, v0, pols)
+        assert em == 0, ev.last_error()
+        st = ev.stats()
+    finally:
+        ev.set_refine_guard(0.8)
+        ev.set_precision(PREC_FP64)
+        for p in pols:
+            ev.free_pol(p)
+    print(f"  degrees {[p.deg for p in pols]}, counts {[len(r[0]) for r in rm]}, "
+          f"retries {st['n_refine_retries']}, reruns {st['n_fp64_reruns']}, "
+          f"extra passes {st['n_refine_extra']}", flush=True)
+    assert st["n_refine_retries"] >= 1 and st["n_fp64_reruns"] == 0
+    for (l64, _, _), (lm, _, res) in zip(r64, rm):
+        assert len(lm) == len(l64) > 0
+        l64s, lms = np.sort(l64), np.sort(lm)
+        assert np.all(np.abs(lms - l64s) <= 1e-10 * np.maximum(1.0, np.abs(l64s)))
+        assert np.all(res < tol)
