@@ -422,6 +422,8 @@ int make_filter_op(double cc, double dd, bool f32, FilterOp* op) {
   }
   if (int e = ensure_a32()) return e;  // no stored diagonal: unshifted FP32 values
   op->A.a32 = c.A.a32;
+  op->A.pval32 = c.A.pval32;  // made with a32 (the view above predates them)
+  op->A.dg32 = c.A.dg32;
   return 0;
 }
 

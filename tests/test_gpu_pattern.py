@@ -7,9 +7,10 @@ and every sum is the one the CSR-indexed kernels form.
 
 Bars:
 * the matrices of the stencil and lattice families are pattern-coded, with
-  16-bit table offsets (3-D Laplacian, a small periodic Anderson lattice)
-  or 32-bit ones (a periodic Anderson lattice whose wrap-around neighbours
-  lie more than 32767 rows away); a random symmetric matrix is not;
+  16-bit table offsets (3-D Laplacian, a small periodic Anderson lattice, a
+  2-D Laplacian with rows that store no diagonal entry) or 32-bit ones (a
+  periodic Anderson lattice whose wrap-around neighbours lie more than
+  32767 rows away); a random symmetric matrix is not;
 * with pattern codes on and off, the blocked ChebAv (the block form of
   SRC/chebpoly.c:519-598) gives the same bits at every width in FP64 and in
   the mixed form (shifted FP32 copy), and so do the FP64 KPM moments
@@ -32,6 +33,17 @@ def make_pol(ev, a, b, lo, hi, max_deg=200):
 
 
 def case_matrix(case):
+    if case == "lap2d_gaps":
+        # a 2-D Laplacian whose every third row stores no diagonal entry: the
+        # FP32 path then runs on the unshifted values and their tables
+        import scipy.sparse as sp
+        from evsl_amd.types import HostCsr
+        S = M.laplacian(60, 60, 1).to_scipy().tolil()
+        for r in range(0, S.shape[0], 3):
+            S[r, r] = 0.0
+        S = sp.csr_matrix(S)
+        S.eliminate_zeros()
+        return HostCsr.from_scipy(S), 2
     if case == "lap3d":
         return M.perturb_diag(M.laplacian(30, 30, 30), 1e-3, seed=3), 2
     if case == "anderson_short":  # wrap-around offsets 24 * 23 * 24 < 32768
@@ -41,7 +53,8 @@ def case_matrix(case):
     return M.random_symmetric(20000, 8, 32, seed=11), None
 
 
-@pytest.mark.parametrize("case", ["lap3d", "anderson_short", "anderson_long", "randsym"])
+@pytest.mark.parametrize("case", ["lap3d", "lap2d_gaps", "anderson_short", "anderson_long",
+                                  "randsym"])
 def test_pattern_codes_same_bits(ev, case):
     A, mode = case_matrix(case)
     lo, hi = M.gershgorin(A)
