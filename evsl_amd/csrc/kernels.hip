@@ -342,6 +342,9 @@ __global__ __launch_bounds__(kWG) void k_gemv_n(size_t n, int k, const double* _
     if (r < n) {
       const double* Vr = V + (size_t)cbase * ld + r;
       int c = 0;
+      // two groups of four columns per trip: eight loads in flight, the
+      // same additions in the same order
+#pragma unroll 2
       for (; c + 4 <= nc; c += 4) {
         a0 += Vr[(size_t)c * ld] * s_h[c];
         a1 += Vr[(size_t)(c + 1) * ld] * s_h[c + 1];
