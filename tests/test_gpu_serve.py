@@ -9,6 +9,8 @@ Bars:
 * FP64 slices equal the drop-in ChebLanNr (SRC/cheblanNr.c) on the same
   matrix, interval, filter and start vector, bit for bit; FP64 DOS requests
   equal kpmdos_ex with the same seed, bit for bit;
+* served slices, FP64 and mixed, match the oracle's ChebLanNr directly
+  (counts equal, eigenvalues within 1e-10 relative);
 * admission keeps the projected memory within the budget, and the requests
   that wait are served in arrival order."""
 import numpy as np
@@ -178,3 +180,31 @@ def test_server_grouped_launches_equal_per_matrix_filters(ev, prec):
         if r[0] == "slice":
             assert len(a[0]) > 0, r
             assert np.all(a[1] < TOL)
+
+
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_MIXED])
+def test_server_slices_match_the_oracle(ev, oracle, prec):
+    """Every slice the request loop serves in the mix, against the oracle's
+    ChebLanNr (the C restatement of SRC/cheblanNr.c) on the same matrix,
+    interval, filter and start vector: the same number of eigenpairs,
+    eigenvalues within 1e-10 relative, residuals below tol."""
+    P = pool()
+    reqs = workload(P)
+    mix, _ = run(ev, P, reqs, prec)
+    for (kind, m, a, b, x), got in zip(reqs, mix):
+        if kind != "slice":
+            continue
+        A, lo, hi = P[m]
+        oracle.set_csr(A)
+        iv = np.array([a, b, lo, hi])
+        pol = oracle.default_pol()
+        pol.damping, pol.thresh_int, pol.thresh_ext, pol.max_deg = 2, 0.8, 0.2, 3000
+        assert oracle.find_pol(iv, pol) == 0
+        e, lam, _, res = oracle.ChebLanNr(iv, MAXIT, TOL, start_vector(P, m, x), pol)
+        oracle.free_pol(pol)
+        assert e == 0
+        gl = np.sort(got[0])
+        print(f"  slice m{m} [{a}, {b}]: served {len(gl)}, oracle {len(lam)}", flush=True)
+        assert len(gl) == len(lam) > 0, (prec, kind, m, a, b)
+        np.testing.assert_allclose(gl, np.sort(lam), rtol=1e-10, atol=1e-13)
+        assert np.all(got[1] < TOL)
