@@ -317,3 +317,77 @@ def test_mixed_stalled_refinement_retried_at_half_guard(ev):
         l64s, lms = np.sort(l64), np.sort(lm)
         assert np.all(np.abs(lms - l64s) <= 1e-10 * np.maximum(1.0, np.abs(l64s)))
         assert np.all(res < tol)
+
+
+def random_trial(trial):
+    """Trial `trial` of the seeded sweep: a matrix without repeated
+    eigenvalues (perturbed 2-D or 3-D Laplacian, Anderson model with W in
+    [4, 20], random symmetric), 2-4 consecutive slices of about 40-150
+    eigenvalues each, cut by spslicer from the KPM density of the whole
+    spectrum (the first trial of each kind whose run starts at slice 0 has
+    an end interval)."""
+    rng = np.random.default_rng(1000 + trial)
+    kind = trial % 4
+    seed = int(rng.integers(1 << 30))
+    if kind == 0:
+        nx, ny = (int(v) for v in rng.integers(70, 140, size=2))
+        A = M.perturb_diag(M.laplacian(nx, ny, 1), 1e-3, seed=seed)
+    elif kind == 1:
+        m = int(rng.integers(16, 24))
+        A = M.perturb_diag(M.laplacian(m, m + 1, m + 2), 1e-3, seed=seed)
+    elif kind == 2:
+        A = M.anderson(int(rng.integers(14, 22)), float(rng.uniform(4.0, 20.0)), seed=seed)
+    else:
+        A = M.random_symmetric(int(rng.integers(5000, 16000)), 8, 32, seed=seed)
+    per = int(rng.integers(40, 150))
+    ns = int(rng.integers(2, 5))
+    nall = max(ns + 1, A.n // per)
+    first = 0 if trial < 4 else int(rng.integers(0, nall - ns + 1))
+    return A, nall, first, ns
+
+
+@pytest.mark.parametrize("trial", range(12))
+def test_mixed_counts_equal_fp64_randomized(ev, trial):
+    """Seeded sweep of the mixed form against FP64 batched (the stopping
+    test of the mixed form -- trace change below 1e-7 max(1, nconv), count of
+    Ritz values above bar unchanged -- differs from the reference's 2e-12):
+    identical per-slice counts, eigenvalues within 1e-10 relative, residuals
+    below tol, on matrices whose eigenvalues are simple."""
+    A, nall, first, ns = random_trial(trial)
+    lo, hi = M.gershgorin(A)
+    assert ev.set_matrix(A) == 0, ev.last_error()
+    ev.set_precision(PREC_FP64)
+    whole = np.array([lo, hi, lo, hi])
+    err, mu, ecnt = ev.kpmdos(300, 1, 40, whole, vec_kind=VEC_RADEMACHER, seed=7)
+    assert err == 0
+    err, sli = ev.spslicer(mu, whole, nall, 10 * A.n)
+    assert err == 0
+    ivs = [np.array([sli[k], sli[k + 1], lo, hi]) for k in range(first, first + ns)]
+    mlan = min(max(5 * (int(1 + A.n / nall) + 2), 300), A.n)
+    pols = []
+    for iv in ivs:
+        pol = ev.default_pol()
+        pol.damping, pol.thresh_int, pol.thresh_ext, pol.max_deg = 2, 0.8, 0.2, 3000
+        assert ev.find_pol(iv, pol) == 0
+        pols.append(pol)
+    v0 = np.random.default_rng(50 + trial).uniform(-1, 1, A.n)
+    tol = 1e-8
+    try:
+        e64, r64 = ev.ChebLanNr_batch(ivs, mlan, tol, v0, pols)
+        assert e64 == 0, ev.last_error()
+        ev.set_precision(PREC_MIXED)
+        em, rm = ev.ChebLanNr_batch(ivs, mlan, tol, v0, pols)
+        assert em == 0, ev.last_error()
+    finally:
+        ev.set_precision(PREC_FP64)
+        for p in pols:
+            ev.free_pol(p)
+    print(f"  trial {trial}: n {A.n}, slices {first}..{first + ns - 1} of {nall}, degrees "
+          f"{[p.deg for p in pols]}, counts {[len(r[0]) for r in r64]}", flush=True)
+    assert sum(len(r[0]) for r in r64) > 0
+    for (l64, _, _), (lm, _, res) in zip(r64, rm):
+        assert len(lm) == len(l64)
+        if len(lm):
+            l64s, lms = np.sort(l64), np.sort(lm)
+            assert np.all(np.abs(lms - l64s) <= 1e-10 * np.maximum(1.0, np.abs(l64s)))
+            assert np.all(res < tol)
