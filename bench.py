@@ -264,16 +264,20 @@ def dense_bench(args):
     """The dense FP64 kernels of the solve at the 100^3 workload's sizes:
     Gram-Schmidt passes (k_gemv_t_partial + k_gemv_n) against k = 128, 300
     and 600 basis vectors of 10^6 rows (the mean basis length of the mixed
-    solve is ~290), the Ritz GEMM V Z (10^6 x 573 x 266: the last basis
-    times the refinement block) and the Gram product U^T U (10^6 x 266).
-    HBM peak 8 TB/s, FP64 peak 78.6 TFLOP/s (vector and matrix alike)."""
+    solve is ~290), the same passes against an FP32 copy of the basis
+    (k_gemv_t_partial_f32 + k_gemv_n_sub_f32, the mixed form's), the Ritz
+    GEMM V Z (10^6 x 573 x 266: the last basis times the refinement block)
+    and the Gram product U^T U (10^6 x 266).  HBM peak 8 TB/s, FP64 peak
+    78.6 TFLOP/s (vector and matrix alike)."""
     from evsl_amd.evsl import EVSL
     ev = EVSL()
     assert ev.start() == 0
     n = 10 ** 6
     out = []
     for kind, k, m, name in [(0, 128, 1, "cgs_pass_k128"), (0, 300, 1, "cgs_pass_k300"),
-                             (0, 600, 1, "cgs_pass_k600"), (1, 573, 266, "ritz_gemm_nn"),
+                             (0, 600, 1, "cgs_pass_k600"), (3, 128, 1, "cgs_pass_f32_k128"),
+                             (3, 300, 1, "cgs_pass_f32_k300"), (3, 600, 1, "cgs_pass_f32_k600"),
+                             (1, 573, 266, "ritz_gemm_nn"),
                              (2, 1, 266, "gram_gemm_tn")]:
         err, ms, by, fl = ev.bench_dense(kind, n, k, m, 10)
         assert err == 0, ev.last_error()

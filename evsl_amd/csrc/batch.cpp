@@ -353,11 +353,17 @@ struct Slice {
   double intv[4];
   polparams* pol;
   DevBuf V;
+  // FP32 copy of the basis read by the Gram-Schmidt passes (empty: they
+  // read V); columns ld32 apart, ld32 even and >= n
+  DevBuf V32;
+  size_t ld32 = 0;
   std::vector<double> dT, eT;
   double beta = 0.0, wn = 0.0, tr0 = 0.0, old2 = 0.0, nrm = 0.0;
   int nwn = 0, kdim = 0, last_nconv = -1;
   bool active = true;
   double* vcol(int k, int n) const { return V.as<double>() + (size_t)k * n; }
+  float* v32col(int k) const { return V32.as<float>() + (size_t)k * ld32; }
+  bool has32() const { return V32.p != nullptr; }
 };
 
 // Z = p(A~) U for the m columns of U (FP64, column-major), in the form
@@ -814,6 +820,48 @@ struct LanczosGroup::Impl {
     sl.f32 = f32;
     sl.state = 1;
     EVSL_HIP(sl.L.V.alloc((size_t)n * (sl.maxit + 1) * sizeof(double)));
+    if (f32 && ctx().fp32_basis) {
+      sl.L.ld32 = ((size_t)n + 63) / 64 * 64;
+      EVSL_HIP(sl.L.V32.alloc(sl.L.ld32 * (sl.maxit + 1) * sizeof(float)));
+    } else {
+      sl.L.V32.release();
+      sl.L.ld32 = 0;
+    }
+    return 0;
+  }
+  // one Gram-Schmidt pass of vnew against the slice's first k + 1 basis
+  // vectors (the FP32 copy when the slice has one); the partials of
+  // ||vnew||^2 go to part, their count to *nparts
+  int cgs_pass(Slice& L, int k, double* vnew, double* part, int* nparts) {
+    const int n = ctx().n;
+    hipStream_t st = ctx().stream;
+    if (L.has32()) {
+      EVSL_HIP(evsl_dev::gemv_t_f32(n, k + 1, L.V32.as<float>(), L.ld32, vnew,
+                                    gt_part.as<double>(), hbuf.as<double>(), st));
+      EVSL_HIP(evsl_dev::gemv_n_sub_nrm_f32(n, k + 1, L.V32.as<float>(), L.ld32,
+                                            hbuf.as<double>(), vnew, part, st));
+      *nparts = evsl_dev::gemv_n_sub_f32_blocks(n);
+      return 0;
+    }
+    EVSL_HIP(evsl_dev::gemv_t(n, k + 1, L.V.as<double>(), n, vnew, gt_part.as<double>(),
+                              hbuf.as<double>(), st));
+    EVSL_HIP(evsl_dev::gemv_n_sub_nrm(n, k + 1, L.V.as<double>(), n, hbuf.as<double>(), vnew,
+                                      part, st));
+    *nparts = (n + evsl_dev::kWG - 1) / evsl_dev::kWG;
+    return 0;
+  }
+  // bytes one Gram-Schmidt pass against k + 1 vectors reads and writes
+  static double cgs_bytes(const Slice& L, int n, int k) {
+    return (L.has32() ? 8.0 : 16.0) * n * (k + 1) + 24.0 * n;
+  }
+  // basis column k: scale by a, and write its FP32 copy when there is one
+  int scal_col(Slice& L, int k, double a) {
+    const int n = ctx().n;
+    if (L.has32()) {
+      EVSL_HIP(evsl_dev::scal_f32copy(n, a, L.vcol(k, n), L.v32col(k), ctx().stream));
+    } else {
+      EVSL_HIP(evsl_dev::scal(n, a, L.vcol(k, n), ctx().stream));
+    }
     return 0;
   }
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -840,6 +888,20 @@ struct LanczosGroup::Impl {
     return 0;
   }
 };
+
+bool mixed_slice_fp32(const polparams* pol) {
+  const Context& c = ctx();
+  const bool capped =
+      c.mixed_capped_fp64 && pol->max_deg > 0 && pol->deg >= pol->max_deg - 1;
+  return !capped && pol->deg >= c.mixed_min_deg && pol->deg <= c.mixed_max_deg;
+}
+
+size_t slice_basis_bytes(int n, int maxit, bool mixed, const polparams* pol) {
+  size_t b = (size_t)n * (maxit + 2) * sizeof(double);
+  if (mixed && ctx().fp32_basis && mixed_slice_fp32(pol))
+    b += ((size_t)n + 63) / 64 * 64 * (maxit + 1) * sizeof(float);
+  return b;
+}
 
 LanczosGroup::LanczosGroup(bool mixed, FILE* fstats) : p_(new Impl) {
   p_->mixed = mixed;
@@ -896,10 +958,7 @@ int LanczosGroup::add(const double* intv, int maxit, double tol, const double* v
   // (deg = max_deg - 1, SRC/chebpoly.c:363-490) separates its interval
   // weakly, its slice tends to end at maxit, and there the FP32-filtered
   // basis leaves pairs just above tol and the slice would be redone in FP64
-  const bool capped =
-      c.mixed_capped_fp64 && pol->max_deg > 0 && pol->deg >= pol->max_deg - 1;
-  const bool f32 =
-      g.mixed && !capped && pol->deg >= c.mixed_min_deg && pol->deg <= c.mixed_max_deg;
+  const bool f32 = g.mixed && mixed_slice_fp32(pol);
   if (g.mixed && !f32) ++c.astats.n_fp64_slices;
   if (int e = g.start(s, f32)) return e;
   return g.ensure(sl.maxit);
@@ -933,7 +992,7 @@ int LanczosGroup::start_done() {
   std::vector<double> nv(4 * g.slots.size());
   EVSL_HIP(hipMemcpy(nv.data(), sc_d, nv.size() * sizeof(double), hipMemcpyDeviceToHost));
   for (int s : g.st_cur) {
-    EVSL_HIP(evsl_dev::scal(n, 1.0 / std::sqrt(nv[4 * s]), g.slots[s]->L.vcol(0, n), c.stream));
+    if (int e = g.scal_col(g.slots[s]->L, 0, 1.0 / std::sqrt(nv[4 * s]))) return e;
     g.slots[s]->state = 2;
   }
   g.st_cur.clear();
@@ -1003,7 +1062,6 @@ int LanczosGroup::step_done() {
     return ms * 1e-3;
   };
   const double eta = 1.0 / std::sqrt(2.0);
-  const int nb_rows = (n + evsl_dev::kWG - 1) / evsl_dev::kWG;
   double* sc_d = g.scal_d.as<double>();
   const std::vector<int> act = g.act_cur;
   if (act.empty()) return 0;
@@ -1025,11 +1083,9 @@ int LanczosGroup::step_done() {
     EVSL_HIP(evsl_dev::axpy_dev_nrm(n, -1.0, sc_d + 4 * s, v, vnew, g.part_of(s), c.stream));
     EVSL_HIP(evsl_dev::sum_scalar(g.part_of(s), evsl_dev::kRedBlocks, sc_d + 4 * s + 1, c.stream));
     // first Gram-Schmidt pass
-    EVSL_HIP(evsl_dev::gemv_t(n, k + 1, L.V.as<double>(), n, vnew, g.gt_part.as<double>(),
-                              g.hbuf.as<double>(), c.stream));
-    EVSL_HIP(evsl_dev::gemv_n_sub_nrm(n, k + 1, L.V.as<double>(), n, g.hbuf.as<double>(), vnew,
-                                      g.part_of(s), c.stream));
-    EVSL_HIP(evsl_dev::sum_scalar(g.part_of(s), nb_rows, sc_d + 4 * s + 2, c.stream));
+    int np = 0;
+    if (int e = g.cgs_pass(L, k, vnew, g.part_of(s), &np)) return e;
+    EVSL_HIP(evsl_dev::sum_scalar(g.part_of(s), np, sc_d + 4 * s + 2, c.stream));
   }
   std::vector<double> hs(4 * g.slots.size());
   EVSL_HIP(hipMemcpy(hs.data(), sc_d, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -1043,7 +1099,7 @@ int LanczosGroup::step_done() {
     L.wn += std::fabs(alpha);
     L.nrm = std::sqrt(hs[4 * s + 2]);
     c.astats.n_reorth_pass++;
-    c.astats.reorth_bytes += 16.0 * n * (k + 1) + 24.0 * n;
+    c.astats.reorth_bytes += Impl::cgs_bytes(L, n, k);
     if (!(L.nrm > eta * old_nrm)) second.push_back(s);
   }
   for (int s : second) {
@@ -1051,13 +1107,11 @@ int LanczosGroup::step_done() {
     Slice& L = sl.L;
     const int k = sl.k;
     double* vnew = L.vcol(k + 1, n);
-    EVSL_HIP(evsl_dev::gemv_t(n, k + 1, L.V.as<double>(), n, vnew, g.gt_part.as<double>(),
-                              g.hbuf.as<double>(), c.stream));
-    EVSL_HIP(evsl_dev::gemv_n_sub_nrm(n, k + 1, L.V.as<double>(), n, g.hbuf.as<double>(), vnew,
-                                      g.part_of(s), c.stream));
-    EVSL_HIP(evsl_dev::sum_scalar(g.part_of(s), nb_rows, sc_d + 4 * s + 3, c.stream));
+    int np = 0;
+    if (int e = g.cgs_pass(L, k, vnew, g.part_of(s), &np)) return e;
+    EVSL_HIP(evsl_dev::sum_scalar(g.part_of(s), np, sc_d + 4 * s + 3, c.stream));
     c.astats.n_reorth_pass++;
-    c.astats.reorth_bytes += 16.0 * n * (k + 1) + 24.0 * n;
+    c.astats.reorth_bytes += Impl::cgs_bytes(L, n, k);
   }
   if (!second.empty()) {
     EVSL_HIP(hipMemcpy(hs.data(), sc_d, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -1082,20 +1136,18 @@ int LanczosGroup::step_done() {
       if (int e = read_scalars(red_scalar(0), 1, d2)) return e;
       double old = std::sqrt(d2[0]), nn = old;
       for (int it = 0; it < 2; ++it) {
-        EVSL_HIP(evsl_dev::gemv_t(n, k + 1, L.V.as<double>(), n, vnew, g.gt_part.as<double>(),
-                                  g.hbuf.as<double>(), c.stream));
-        EVSL_HIP(evsl_dev::gemv_n_sub_nrm(n, k + 1, L.V.as<double>(), n, g.hbuf.as<double>(),
-                                          vnew, red_part(), c.stream));
-        EVSL_HIP(evsl_dev::sum_scalar(red_part(), nb_rows, red_scalar(1), c.stream));
+        int np = 0;
+        if (int e = g.cgs_pass(L, k, vnew, red_part(), &np)) return e;
+        EVSL_HIP(evsl_dev::sum_scalar(red_part(), np, red_scalar(1), c.stream));
         if (int e = read_scalars(red_scalar(1), 1, d2)) return e;
         nn = std::sqrt(d2[0]);
         if (nn > eta * old) break;
         old = nn;
       }
-      EVSL_HIP(evsl_dev::scal(n, 1.0 / nn, vnew, c.stream));
+      if (int e = g.scal_col(L, k + 1, 1.0 / nn)) return e;
       L.beta = 0.0;
     } else {
-      EVSL_HIP(evsl_dev::scal(n, 1.0 / L.beta, vnew, c.stream));
+      if (int e = g.scal_col(L, k + 1, 1.0 / L.beta)) return e;
     }
     L.eT[k] = L.beta;
   }
@@ -1156,6 +1208,7 @@ int LanczosGroup::step_done() {
     L.last_nconv = nconv;
     if (done || k == sl.maxit - 1) {
       L.kdim = k + 1;
+      L.V32.release();  // the Ritz vectors and the refinement use the FP64 basis
       const double tr = evsl_timer();
       if (fstats) std::fprintf(fstats, "slice %d done at k %d\n", s, k);
       SliceResult* o = sl.out;

@@ -330,6 +330,16 @@ hipError_t gemv_n_sub_nrm(size_t n, int k, const double* V, size_t ld, const dou
 // y = V h (no subtraction)
 hipError_t gemv_n(size_t n, int k, const double* V, size_t ld, const double* h, double* y,
                   hipStream_t st);
+// The same pass against an FP32 copy of the basis (ld even, columns 8-byte
+// aligned), FP64 products and sums; gemv_n_sub_nrm_f32 leaves
+// gemv_n_sub_f32_blocks(n) partials of ||w||^2 (one per 512 rows).
+hipError_t gemv_t_f32(size_t n, int k, const float* V, size_t ld, const double* w,
+                      double* part, double* h, hipStream_t st);
+int gemv_n_sub_f32_blocks(size_t n);
+hipError_t gemv_n_sub_nrm_f32(size_t n, int k, const float* V, size_t ld, const double* h,
+                              double* w, double* part, hipStream_t st);
+// x *= a, x32 = (float)x
+hipError_t scal_f32copy(size_t n, double a, double* x, float* x32, hipStream_t st);
 
 // C(n x m) = A(n x k) * B(k x m), all column-major, FP64
 hipError_t gemm_nn(size_t n, int m, int k, const double* A, size_t lda, const double* B,

@@ -120,6 +120,9 @@ struct Context {
   int mixed_min_deg = 32;    // mixed form: filters of degree outside
   int mixed_max_deg = 20000; // [mixed_min_deg, mixed_max_deg] run in FP64
   bool mixed_capped_fp64 = true;  // mixed form: capped filters run in FP64
+  // mixed form: the Gram-Schmidt passes of a slice with an FP32 filter read
+  // an FP32 copy of its Lanczos basis (FP64 products and sums)
+  bool fp32_basis = true;
   bool spmm_tuning = true;   // filter launches go through the matrix's layout tuner
   bool pattern_codes = true; // matrices built from now on use pattern codes when they can
   // the request loop filters the slices of all its matrices in grouped
@@ -234,6 +237,13 @@ struct SliceResult {
 // between steps joins with the next one.  pol and v0 must stay valid until
 // the slice is done (a mixed slice whose refinement stalls restarts from v0
 // in FP64).
+// the mixed form's rule for a slice's filter: FP32 unless its degree lies
+// outside the mixed degree range or it is a capped filter run in FP64
+bool mixed_slice_fp32(const polparams* pol);
+// device bytes a slice's Lanczos basis takes (FP64, plus the FP32 copy of a
+// mixed slice that filters in FP32)
+size_t slice_basis_bytes(int n, int maxit, bool mixed, const polparams* pol);
+
 class LanczosGroup {
  public:
   LanczosGroup(bool mixed, FILE* fstats);

@@ -258,6 +258,17 @@ __global__ void k_f32_to_f64(size_t n, const float* __restrict__ x, double* __re
     y[i] = (double)x[i];
 }
 
+// x *= a and x32 = (float)x (a basis column and its FP32 copy)
+__global__ void k_scal_f32copy(size_t n, double a, double* __restrict__ x,
+                               float* __restrict__ x32) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double v = x[i] * a;
+    x[i] = v;
+    x32[i] = (float)v;
+  }
+}
+
 // ---------------------------------------------------------------------
 // tall-skinny Gram-Schmidt kernels
 // ---------------------------------------------------------------------
@@ -369,6 +380,132 @@ __global__ __launch_bounds__(kWG) void k_gemv_n(size_t n, int k, const double* _
     d = block_sum(d, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = d;
   }
+}
+
+// The same two passes against an FP32 copy of the basis (V32, ld32 even,
+// columns 8-byte aligned), products and sums in FP64: each thread takes a
+// pair of adjacent rows with one 8-byte load per column, so a wave moves
+// 512 bytes per load as the FP64 kernels do, and the basis is read at half
+// their bytes.  A row pair that ends past n (odd n) takes its one row alone;
+// rows >= n of V32 (padding) are never read.
+__global__ __launch_bounds__(kWG) void k_gemv_t_partial_f32(size_t n, int k,
+                                                            const float* __restrict__ V,
+                                                            size_t ld,
+                                                            const double* __restrict__ w,
+                                                            double* __restrict__ part, int nch,
+                                                            int ncg) {
+  __shared__ double sh[4][kGtCols];
+  const int lb = xcd_remap(blockIdx.x, nch * ncg);
+  const int ch = lb / ncg;
+  const int c0 = (lb % ncg) * kGtCols;
+  const int nc = min(kGtCols, k - c0);
+  double acc[kGtCols];
+#pragma unroll
+  for (int c = 0; c < kGtCols; ++c) acc[c] = 0.0;
+  const size_t rbeg = (size_t)ch * kGtRows;  // even
+  const size_t rend = min(n, rbeg + kGtRows);
+  const float* Vc = V + (size_t)c0 * ld;
+  for (size_t r = rbeg + 2 * threadIdx.x; r < rend; r += 2 * kWG) {
+    const double w0 = w[r];
+    if (r + 1 < rend) {
+      const double w1 = w[r + 1];
+      if (nc == kGtCols) {
+#pragma unroll
+        for (int c = 0; c < kGtCols; ++c) {
+          const float2 v = *reinterpret_cast<const float2*>(Vc + (size_t)c * ld + r);
+          acc[c] += (double)v.x * w0;
+          acc[c] += (double)v.y * w1;
+        }
+      } else {
+        for (int c = 0; c < nc; ++c) {
+          const float2 v = *reinterpret_cast<const float2*>(Vc + (size_t)c * ld + r);
+          acc[c] += (double)v.x * w0;
+          acc[c] += (double)v.y * w1;
+        }
+      }
+    } else {
+      for (int c = 0; c < nc; ++c) acc[c] += (double)Vc[(size_t)c * ld + r] * w0;
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < kGtCols; ++c) {
+    const double v = wave_sum(acc[c]);
+    if (lane == 0) sh[wid][c] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < nc) {
+    const int c = threadIdx.x;
+    part[(size_t)(c0 + c) * nch + ch] = sh[0][c] + sh[1][c] + sh[2][c] + sh[3][c];
+  }
+}
+
+// w -= V32 h; part[blk] = partial ||w||^2 over the block's 2 kWG rows
+__global__ __launch_bounds__(kWG) void k_gemv_n_sub_f32(size_t n, int k,
+                                                        const float* __restrict__ V, size_t ld,
+                                                        const double* __restrict__ h,
+                                                        double* __restrict__ w,
+                                                        double* __restrict__ part) {
+  __shared__ double s_h[kHTile];
+  __shared__ double sh[4];
+  const size_t r = 2 * ((size_t)blockIdx.x * kWG + threadIdx.x);
+  const bool pair = r + 1 < n;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;  // row r, columns c .. c + 3
+  double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;  // row r + 1
+  for (int cbase = 0; cbase < k; cbase += kHTile) {
+    const int nc = min(kHTile, k - cbase);
+    __syncthreads();
+    for (int c = threadIdx.x; c < nc; c += kWG) s_h[c] = h[cbase + c];
+    __syncthreads();
+    if (pair) {
+      const float* Vr = V + (size_t)cbase * ld + r;
+      int c = 0;
+      // two groups of four columns per trip: eight 8-byte loads in flight
+#pragma unroll 2
+      for (; c + 4 <= nc; c += 4) {
+        const float2 v0 = *reinterpret_cast<const float2*>(Vr + (size_t)c * ld);
+        const float2 v1 = *reinterpret_cast<const float2*>(Vr + (size_t)(c + 1) * ld);
+        const float2 v2 = *reinterpret_cast<const float2*>(Vr + (size_t)(c + 2) * ld);
+        const float2 v3 = *reinterpret_cast<const float2*>(Vr + (size_t)(c + 3) * ld);
+        a0 += (double)v0.x * s_h[c];
+        a1 += (double)v1.x * s_h[c + 1];
+        a2 += (double)v2.x * s_h[c + 2];
+        a3 += (double)v3.x * s_h[c + 3];
+        b0 += (double)v0.y * s_h[c];
+        b1 += (double)v1.y * s_h[c + 1];
+        b2 += (double)v2.y * s_h[c + 2];
+        b3 += (double)v3.y * s_h[c + 3];
+      }
+      for (; c < nc; ++c) {
+        const float2 v0 = *reinterpret_cast<const float2*>(Vr + (size_t)c * ld);
+        a0 += (double)v0.x * s_h[c];
+        b0 += (double)v0.y * s_h[c];
+      }
+    } else if (r < n) {
+      const float* Vr = V + (size_t)cbase * ld + r;
+      int c = 0;
+      for (; c + 4 <= nc; c += 4) {
+        a0 += (double)Vr[(size_t)c * ld] * s_h[c];
+        a1 += (double)Vr[(size_t)(c + 1) * ld] * s_h[c + 1];
+        a2 += (double)Vr[(size_t)(c + 2) * ld] * s_h[c + 2];
+        a3 += (double)Vr[(size_t)(c + 3) * ld] * s_h[c + 3];
+      }
+      for (; c < nc; ++c) a0 += (double)Vr[(size_t)c * ld] * s_h[c];
+    }
+  }
+  double d = 0.0;
+  if (r < n) {
+    const double v = w[r] - ((a0 + a1) + (a2 + a3));
+    w[r] = v;
+    d = v * v;
+  }
+  if (pair) {
+    const double v = w[r + 1] - ((b0 + b1) + (b2 + b3));
+    w[r + 1] = v;
+    d += v * v;
+  }
+  d = block_sum(d, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = d;
 }
 
 // ---------------------------------------------------------------------
@@ -1153,6 +1290,32 @@ hipError_t gemv_n_sub_nrm(size_t n, int k, const double* V, size_t ld, const dou
                           double* w, double* part, hipStream_t st) {
   const unsigned g = (unsigned)((n + kWG - 1) / kWG);
   hipLaunchKernelGGL(k_gemv_n<true>, dim3(g), dim3(kWG), 0, st, n, k, V, ld, h, w, part);
+  return hipGetLastError();
+}
+
+hipError_t gemv_t_f32(size_t n, int k, const float* V, size_t ld, const double* w,
+                      double* part, double* h, hipStream_t st) {
+  if (k <= 0) return hipSuccess;
+  const int nch = gemv_t_chunks(n);
+  const int ncg = (k + kGtCols - 1) / kGtCols;
+  hipLaunchKernelGGL(k_gemv_t_partial_f32, dim3((unsigned)nch * ncg), dim3(kWG), 0, st, n, k,
+                     V, ld, w, part, nch, ncg);
+  hipLaunchKernelGGL(k_gemv_t_reduce, dim3(k), dim3(64), 0, st, part, nch, h);
+  return hipGetLastError();
+}
+
+int gemv_n_sub_f32_blocks(size_t n) { return (int)((n + 2 * kWG - 1) / (2 * kWG)); }
+
+hipError_t gemv_n_sub_nrm_f32(size_t n, int k, const float* V, size_t ld, const double* h,
+                              double* w, double* part, hipStream_t st) {
+  hipLaunchKernelGGL(k_gemv_n_sub_f32, dim3((unsigned)gemv_n_sub_f32_blocks(n)), dim3(kWG), 0,
+                     st, n, k, V, ld, h, w, part);
+  return hipGetLastError();
+}
+
+hipError_t scal_f32copy(size_t n, double a, double* x, float* x32, hipStream_t st) {
+  hipLaunchKernelGGL(k_scal_f32copy, dim3(grid_for(n, kWG, 4096)), dim3(kWG), 0, st, n, a, x,
+                     x32);
   return hipGetLastError();
 }
 

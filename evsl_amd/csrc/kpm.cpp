@@ -364,18 +364,26 @@ int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, doub
 int evsl_amd_bench_dense(int kind, int n, int k, int m, int reps, double* ms_per_rep,
                          double* bytes, double* flops) {
   if (int e = ensure_started()) return e;
-  if (kind < 0 || kind > 2 || n <= 0 || k <= 0 || m <= 0 || reps <= 0 || !ms_per_rep ||
+  if (kind < 0 || kind > 3 || n <= 0 || k <= 0 || m <= 0 || reps <= 0 || !ms_per_rep ||
       !bytes || !flops)
     return EVSL_ERR_ARG;
   Context& c = ctx();
   const size_t nn = (size_t)n;
-  DevBuf A, B, C, part, h;
+  DevBuf A, B, C, part, h, A32;
   // kind 0: A = V (n x k), B = w (n); kind 1: A = V, B = Z (k x m), C = n x m;
-  // kind 2: A = U (n x m), C = m x m
+  // kind 2: A = U (n x m), C = m x m; kind 3: kind 0 with A32 = (float)V,
+  // columns ld32 apart
   const size_t acols = kind == 2 ? (size_t)m : (size_t)k;
+  const size_t ld32 = (nn + 63) / 64 * 64;
   EVSL_HIP(A.alloc(nn * acols * sizeof(double)));
   EVSL_HIP(evsl_dev::rm_random(nn * acols, 1, 1, 17, 0, 1, A.as<double>(), c.stream));
-  if (kind == 0) {
+  if (kind == 3) {
+    EVSL_HIP(A32.alloc(ld32 * acols * sizeof(float)));
+    for (size_t j = 0; j < acols; ++j)
+      EVSL_HIP(evsl_dev::f64_to_f32(nn, A.as<double>() + j * nn, A32.as<float>() + j * ld32,
+                                    c.stream));
+  }
+  if (kind == 0 || kind == 3) {
     EVSL_HIP(B.alloc(nn * sizeof(double)));
     EVSL_HIP(evsl_dev::rm_random(nn, 1, 1, 18, 0, 1, B.as<double>(), c.stream));
     // V^T w partials (chunks x k), and one norm partial per 256 rows
@@ -392,6 +400,15 @@ int evsl_amd_bench_dense(int kind, int n, int k, int m, int reps, double* ms_per
     EVSL_HIP(part.alloc(evsl_dev::gemm_tn_part_elems(nn, m, m) * sizeof(double)));
   }
   auto once = [&]() -> hipError_t {
+    if (kind == 3) {
+      hipError_t e = evsl_dev::gemv_t_f32(nn, k, A32.as<float>(), ld32, B.as<double>(),
+                                          part.as<double>(), h.as<double>(), c.stream);
+      if (e != hipSuccess) return e;
+      e = evsl_dev::scal(k, 1e-30, h.as<double>(), c.stream);
+      if (e != hipSuccess) return e;
+      return evsl_dev::gemv_n_sub_nrm_f32(nn, k, A32.as<float>(), ld32, h.as<double>(),
+                                          B.as<double>(), part.as<double>(), c.stream);
+    }
     if (kind == 0) {
       hipError_t e = evsl_dev::gemv_t(nn, k, A.as<double>(), nn, B.as<double>(),
                                       part.as<double>(), h.as<double>(), c.stream);
@@ -422,8 +439,9 @@ int evsl_amd_bench_dense(int kind, int n, int k, int m, int reps, double* ms_per
   (void)hipEventDestroy(e1);
   *ms_per_rep = ms / reps;
   const double dn = (double)n, dk = (double)k, dm = (double)m;
-  if (kind == 0) {
-    *bytes = 16.0 * dn * dk + 24.0 * dn;  // V twice, w in, out, in
+  if (kind == 0 || kind == 3) {
+    // V twice (8 or 4 bytes an entry), w in, out, in
+    *bytes = (kind == 0 ? 16.0 : 8.0) * dn * dk + 24.0 * dn;
     *flops = 4.0 * dn * dk;
   } else if (kind == 1) {
     *bytes = 8.0 * (dn * dk + dn * dm + dk * dm);

@@ -698,6 +698,10 @@ int evsl_amd_set_mixed_capped_fp64(int on) {
   ctx().mixed_capped_fp64 = on != 0;
   return 0;
 }
+int evsl_amd_set_fp32_basis(int on) {
+  ctx().fp32_basis = on != 0;
+  return 0;
+}
 int evsl_amd_set_mixed_degree_range(int min_deg, int max_deg) {
   if (min_deg < 0 || max_deg < min_deg) return EVSL_ERR_ARG;
   ctx().mixed_min_deg = min_deg;

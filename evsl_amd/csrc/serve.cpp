@@ -5,7 +5,8 @@
 // an interval of a registered matrix) and DOS requests (kpmdos with seeded
 // device vectors).  Each evsl_amd_server_step:
 //   1. admits queued requests in arrival order while their projected memory
-//      (the Lanczos basis n x (maxit+1) x 8 B, or the KPM vector blocks) fits
+//      (the Lanczos basis n x (maxit+1) x 8 B and, for a mixed slice that
+//      filters in FP32, its FP32 copy, or the KPM vector blocks) fits
 //      the budget;
 //   2. advances, per matrix, every running slice by one filtered Lanczos step
 //      in one blocked pass (LanczosGroup: a slice admitted now joins at this
@@ -463,7 +464,7 @@ int evsl_amd_server_submit_slice(evslAmdServer* s, int mat, double a, double b, 
   if (int e = as_device_vector(vinit, m.M.n, tmp, &v0)) return e;
   EVSL_HIP(q->v0.alloc((size_t)m.M.n * sizeof(double)));
   EVSL_HIP(hipMemcpy(q->v0.p, v0, (size_t)m.M.n * sizeof(double), hipMemcpyDeviceToDevice));
-  q->mem = (size_t)m.M.n * (q->maxit + 2) * sizeof(double);
+  q->mem = slice_basis_bytes(m.M.n, q->maxit, s->mixed, &q->pol);
   s->queue.push_back(q->id);
   s->reqs.push_back(std::move(q));
   return s->reqs.back()->id;

@@ -45,7 +45,7 @@ EXPORTS = [
     "evsl_amd_spmm_vec", "evsl_amd_spmm_index_bytes", "evsl_amd_spmm_index_mode",
     "evsl_amd_set_pattern_codes", "evsl_amd_pattern_codes", "evsl_amd_cheb_av_cols",
     "evsl_amd_cheb_av_cols_noalloc", "evsl_amd_set_grouped_filter",
-    "evsl_amd_set_mixed_capped_fp64",
+    "evsl_amd_set_mixed_capped_fp64", "evsl_amd_set_fp32_basis",
     "evsl_amd_spmm_streaming", "kpmdos_batch",
     "evsl_amd_server_create", "evsl_amd_server_destroy", "evsl_amd_server_add_matrix",
     "evsl_amd_server_submit_slice", "evsl_amd_server_submit_dos", "evsl_amd_server_step",
@@ -162,6 +162,7 @@ class EVSL:
             "evsl_amd_cheb_av_cols_noalloc": (I, [I, pp, VP, VP, I]),
             "evsl_amd_set_grouped_filter": (I, [I]),
             "evsl_amd_set_mixed_capped_fp64": (I, [I]),
+            "evsl_amd_set_fp32_basis": (I, [I]),
             "ChebLanNr_batch": (I, [I, dp, I, D, dp, pp, ip, ddp, ddp, ddp, VP]),
         }
         for name, (res, args) in sig.items():
@@ -416,6 +417,9 @@ class EVSL:
 
     def set_refine_widen(self, f):
         return self.lib.evsl_amd_set_refine_widen(f)
+
+    def set_fp32_basis(self, on):
+        return self.lib.evsl_amd_set_fp32_basis(int(on))
 
     def stats(self):
         s = AmdStats()

@@ -82,6 +82,13 @@ int evsl_amd_set_mixed_degree_range(int min_deg, int max_deg);
 /* MIXED mode: whether a capped filter (deg = max_deg - 1) runs in FP64 (1,
  * the default) or, like the others, in FP32 (0) */
 int evsl_amd_set_mixed_capped_fp64(int on);
+/* MIXED mode: whether the Gram-Schmidt passes of a slice that filters in
+ * FP32 read an FP32 copy of its Lanczos basis (1, the default) or the FP64
+ * basis (0).  The copy is written beside each new basis vector (4 bytes per
+ * entry more memory); the projections V^T w and w - V h keep FP64 products
+ * and sums, so a pass reads half the bytes.  The Ritz vectors and the
+ * refinement use the FP64 basis either way. */
+int evsl_amd_set_fp32_basis(int on);
 
 int evsl_amd_get_stats(evslAmdStats *s);
 void evsl_amd_reset_stats(void);
@@ -211,8 +218,9 @@ int evsl_amd_bench_filter(int B, int prec, int mode, int nsteps, double cc, doub
  * (n rows), reps times between HIP events: kind 0 one Gram-Schmidt pass
  * against k basis vectors (V^T w, then w -= V h with the norm partials),
  * kind 1 the Ritz-vector GEMM V (n x k) Z (k x m), kind 2 the Gram product
- * U^T U of an n x m block.  Returns the mean time and the algorithmic bytes
- * and flops of one repetition. */
+ * U^T U of an n x m block, kind 3 the kind 0 pass against an FP32 copy of V
+ * (the mixed form's passes).  Returns the mean time and the algorithmic
+ * bytes and flops of one repetition. */
 int evsl_amd_bench_dense(int kind, int n, int k, int m, int reps, double *ms_per_rep,
                          double *bytes, double *flops);
 

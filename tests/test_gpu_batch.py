@@ -391,3 +391,51 @@ def test_mixed_counts_equal_fp64_randomized(ev, trial):
             l64s, lms = np.sort(l64), np.sort(lm)
             assert np.all(np.abs(lms - l64s) <= 1e-10 * np.maximum(1.0, np.abs(l64s)))
             assert np.all(res < tol)
+
+
+@pytest.mark.parametrize("dims", [(27, 29, 31), (30, 30, 30)])
+def test_fp32_basis_copy_keeps_results(ev, dims):
+    """Mixed form with the Gram-Schmidt passes reading the FP32 copy of the
+    basis (evsl_amd_set_fp32_basis, the default) against the same solve with
+    the passes on the FP64 basis, and against FP64 batched: identical
+    per-slice counts, eigenvalues within 1e-10 relative, residuals below tol,
+    and the passes' byte count (8 bytes per basis entry instead of 16)
+    follows.  An odd n (27 x 29 x 31) takes the kernels' single-row tails."""
+    A = M.perturb_diag(M.laplacian(*dims), 1e-3, seed=17)
+    lo, hi = M.gershgorin(A)
+    ivs, pols, mlan = setup_slices(ev, A, 0.45, 0.6, lo, hi, 4)
+    v0 = np.random.default_rng(8).uniform(-1, 1, A.n)
+    tol = 1e-8
+    out, st = {}, {}
+    try:
+        ev.set_precision(PREC_FP64)
+        e, out["fp64"] = ev.ChebLanNr_batch(ivs, mlan, tol, v0, pols)
+        assert e == 0, ev.last_error()
+        ev.set_precision(PREC_MIXED)
+        for on in (0, 1):
+            ev.set_fp32_basis(on)
+            ev.reset_stats()
+            e, out[on] = ev.ChebLanNr_batch(ivs, mlan, tol, v0, pols)
+            assert e == 0, ev.last_error()
+            st[on] = ev.stats()
+    finally:
+        ev.set_fp32_basis(1)
+        ev.set_precision(PREC_FP64)
+        for p in pols:
+            ev.free_pol(p)
+    n = A.n
+    for on in (0, 1):
+        assert st[on]["n_fp64_slices"] == 0 and st[on]["n_lanczos"] > 0
+    # a pass against k + 1 vectors counts (16 or 8) n (k + 1) + 24 n bytes
+    r = st[1]["reorth_bytes"] - 24.0 * n * st[1]["n_reorth_pass"]
+    r0 = st[0]["reorth_bytes"] - 24.0 * n * st[0]["n_reorth_pass"]
+    assert 0.4 < r / r0 < 0.6
+    print(f"  n {n}: counts {[len(o[0]) for o in out[1]]}, steps {st[0]['n_lanczos']} / "
+          f"{st[1]['n_lanczos']}, basis bytes ratio {r / r0:.3f}, reorth "
+          f"{st[0]['t_reorth']:.3f} / {st[1]['t_reorth']:.3f} s", flush=True)
+    for (l64, _, _), (l0, _, r0_), (l1, _, r1_) in zip(out["fp64"], out[0], out[1]):
+        assert len(l64) == len(l0) == len(l1) > 0
+        s64 = np.sort(l64)
+        for lm, rm in ((l0, r0_), (l1, r1_)):
+            assert np.all(np.abs(np.sort(lm) - s64) <= 1e-10 * np.maximum(1.0, np.abs(s64)))
+            assert np.all(rm < tol)
