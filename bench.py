@@ -628,8 +628,8 @@ def c3_bench(args):
     KPM (Mdeg 300, 40 Rademacher) -> spslicer into 16 slices -> find_pol
     (damping 2, thresholds 0.8/0.2, max degree 10000) -> the batched solver in
     args.form (mixed or batch64), tol 1e-8.  Per-slice eigenvalues and
-    residuals go to gpurun_out/c3_<form>.npz; the solver's per-step log to
-    gpurun_out/c3_<form>_stats.txt."""
+    residuals go to <out-dir>/c3_<form>.npz; the solver's per-step log to
+    <out-dir>/c3_<form>_stats.txt."""
     from evsl_amd import matrices as M
     from evsl_amd.evsl import EVSL, PREC_FP64, PREC_MIXED, VEC_RADEMACHER
     A = M.anderson(80, 16.5, seed=2024)
@@ -659,13 +659,12 @@ def c3_bench(args):
     t_setup = time.perf_counter() - t0
     log(f"[bench] c3 {args.form}: ecnt {ecnt:.0f}, mlan {mlan}, degrees "
         f"{[int(p.deg) for p in pols]} (setup {t_setup:.1f}s)")
-    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+    os.makedirs(args.out_dir, exist_ok=True)
     libc = ctypes.CDLL(None)
     libc.fopen.restype = ctypes.c_void_p
     libc.fopen.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
     libc.fclose.argtypes = [ctypes.c_void_p]
-    fst = libc.fopen(os.path.join(ROOT, "gpurun_out", f"c3_{args.form}_stats.txt").encode(),
-                     b"w")
+    fst = libc.fopen(os.path.join(args.out_dir, f"c3_{args.form}_stats.txt").encode(), b"w")
     v0 = np.random.default_rng(11).uniform(-1.0, 1.0, A.n)
     ev.reset_stats()
     t0 = time.perf_counter()
@@ -674,7 +673,7 @@ def c3_bench(args):
     libc.fclose(fst)
     assert err == 0, ev.last_error()
     lams = [o[0] for o in outs]
-    np.savez(os.path.join(ROOT, "gpurun_out", f"c3_{args.form}.npz"),
+    np.savez(os.path.join(args.out_dir, f"c3_{args.form}.npz"),
              *[np.sort(l) for l in lams], res=np.concatenate([o[2] for o in outs]))
     st = ev.stats()
     print(json.dumps({"workload": "3-D Anderson 80^3 periodic, W 16.5, [-0.05, 0.05], 16 slices",
@@ -748,7 +747,9 @@ def main():
     p.add_argument("--c3", action="store_true",
                    help="run the precision-form workload instead (3-D Anderson 80^3, 16 slices "
                         "of [-0.05, 0.05]) in --form mixed or batch64, dumping eigenvalues "
-                        "under gpurun_out/ (not the headline line)")
+                        "under --out-dir (not the headline line)")
+    p.add_argument("--out-dir", default=os.path.join(ROOT, "bench_out"),
+                   help="folder for the --c3 eigenvalue dumps and solver log")
     p.add_argument("--serve", action="store_true",
                    help="time the request loop instead: a Poisson stream of slice and DOS "
                         "requests over a resident matrix pool (not the headline line)")
@@ -771,8 +772,8 @@ def main():
         cpu_worker(json.loads(args.cpu_worker))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
-                                   "only (slices shard with no communication)",
+        print(json.dumps({"error": "this implementation runs on one GPU; multi-GPU sharding is "
+                                   "a design only (slices shard with no communication)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     if args.kpm_batch:

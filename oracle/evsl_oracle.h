@@ -2,8 +2,8 @@
  * evsl_oracle.h -- TEST INFRASTRUCTURE ONLY.
  *
  * A plain-C CPU restatement of the EVSL 1.1.1 filtered-Lanczos / KPM hot path,
- * used as the checker by tests/, __graft_entry__.smoke() and the cpu_baseline
- * leg of bench.py.  It is never linked into, loaded by or called from the
+ * used to check the library's results by tests/, __graft_entry__.smoke() and
+ * the cpu_baseline leg of bench.py.  It is never linked into, loaded by or called from the
  * product library (evsl_amd/libevsl_amd.so).
  *
  * Each ora_* function restates the reference function named beside it

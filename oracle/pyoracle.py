@@ -1,8 +1,9 @@
 """ctypes bindings of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product package evsl_amd
-never imports this module.
+cpu_baseline leg of bench.py, to check the library's results (and, in the
+cpu_baseline leg, to time the CPU path).  The product package evsl_amd never
+imports this module.
 """
 import ctypes as C
 import os

@@ -1,7 +1,7 @@
 """GPU parity: the HIP path, called through the C ABI, against the CPU oracle
 on the same matrices, intervals, filters and libc rand() streams.
 
-Bars (the north-star tolerances): identical per-slice eigenvalue counts,
+Bars: identical per-slice eigenvalue counts,
 FP64 eigenvalues within 1e-10 relative, residuals below tol, KPM moments
 within 1e-12 (absolute; mu_0 = 1/pi), slice boundaries identical."""
 import ctypes as C
